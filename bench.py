@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Benchmark: binned chaos-game iterations/s of the fast engine on MI355X.
+
+Workload (default cfg2): 8 xforms + final xform + post-affines, 3-5 of 12
+variations per xform, 7680x4320 RGBA FP32 accumulator (3840x2160 at
+supersample 2), 2^24 particles, 20 unbinned iterations, then 2^32 binned
+samples.  A step is one draw_to_bins call of 16 passes over all particles
+(2^28 iterations), so the default 16 timed steps are the whole 2^32-sample
+job.  Inputs (flame, particles, histogram) are resident in HBM before the
+timed region.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="cfg2", choices=["cfg1", "cfg2", "cfg3"])
+    ap.add_argument("--hist-mode", type=int, default=1, help="1 quad, 2 wave-aggregated, 3 naive")
+    ap.add_argument("--waves", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-probe", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(wl, budget_s):
+    """The CPU oracle (FP64 chaos game, C threads) on a bounded sample of the workload."""
+    from oracle import pyoracle as po
+    threads = max(1, min(16, os.cpu_count() or 1))
+    W, H = wl.width, wl.height
+    # calibrate on a small run, then size the sample to ~budget_s seconds
+    parts, iters, fuse = 1 << 12, 64, 20
+    t0 = time.perf_counter()
+    _, n0 = po.chaos_f64(wl.flame, W // 8, H // 8, parts, fuse, iters, seed=3, threads=threads)
+    dt0 = time.perf_counter() - t0
+    rate0 = (parts * (iters + fuse)) / max(dt0, 1e-6)
+    target = max(1, int(rate0 * budget_s * 0.8))
+    iters = 256
+    parts = max(threads * 64, target // (iters + fuse))
+    t0 = time.perf_counter()
+    _, n = po.chaos_f64(wl.flame, W // 4, H // 4, parts, fuse, iters, seed=4, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "binned samples/s", "cores": threads, "kind": "port",
+            "sample": f"{wl.name} flame, FP64 C oracle (per-particle chaos game), {parts} particles x "
+                      f"({fuse} unbinned + {iters} binned) iterations on {threads} threads into a "
+                      f"{W // 4}x{H // 4} histogram; {n} binned in {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design note, see DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+    import refrakt_amd as rf
+    from refrakt_amd import configs
+
+    wl = configs.WORKLOADS[args.config]()
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    ctx = rf.Context(wl.particles, wl.temporal_samples, 1024, seed=12345, engine=rf.RF_ENGINE_FAST,
+                     hist_mode=args.hist_mode, block_waves=args.waves)
+    fh = ctx.flame(wl.flame)
+    hist = ctx.histogram(wl.width, wl.height)
+    fh.warmup(wl.warmup_passes)
+    for _ in range(args.warmup):
+        fh.draw_to_bins(hist, wl.draw_passes)
+    hist.zero()
+    torch.cuda.synchronize()
+    drawn = 0
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        drawn += fh.draw_to_bins(hist, wl.draw_passes)
+        kernel_ms.append(ctx.last_draw_ms())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = drawn / wall
+    attempted = wl.particles * wl.draw_passes * args.steps
+
+    # L2 atomic ceiling: 16-byte RGBA requests (4 lanes per bin) scattered over
+    # an L2-resident footprint (2^16 bins = 1 MiB), measured live.
+    probe = None
+    if not args.no_probe:
+        samples = 1 << 30
+        ms = ctx.atomic_probe(1, 1 << 16, samples)
+        probe = samples / (ms / 1e3)
+    k_avg = float(np.mean(kernel_ms))
+    per_launch_samples = drawn / args.steps
+    achieved = per_launch_samples * 4 / (k_avg / 1e3) / 1e9          # G atomic dword-adds/s
+    peak = probe * 4 / 1e9 if probe else None
+    roofline = {"bound": "l2_atomic", "achieved": round(achieved, 2), "peak": round(peak, 2) if peak else None,
+                "unit": "G atomic f32 adds/s", "frac": round(achieved / peak, 4) if peak else None,
+                "traffic": None, "kernel": "k_fast", "kernel_avg_ms": round(k_avg, 3),
+                "note": "one binned sample = one RGBA FP32 add = 4 f32 atomics in one 16-byte L2 request; "
+                        "peak = live probe of the same request shape on a 1 MiB footprint"}
+    # DE + tonemap of the finished histogram (HBM-bound pass), timed separately
+    out = torch.empty((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
+    de = rf.DEParams(wl.flame.estimator_radius, wl.flame.estimator_min, wl.flame.estimator_curve)
+    tm = rf.TonemapParams(wl.flame.gamma, wl.flame.brightness, wl.flame.vibrancy, 1e-4)
+    ctx.density_tonemap(hist, de, tm, out.data_ptr())
+    torch.cuda.synchronize()
+    td = time.perf_counter()
+    ctx.density_tonemap(hist, de, tm, out.data_ptr())
+    torch.cuda.synchronize()
+    de_ms = (time.perf_counter() - td) * 1e3
+
+    line = {
+        "metric": "binned chaos-game iterations/sec at 4K",
+        "value": value,
+        "unit": "binned samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded flame, configs.py)",
+        "config": {"workload": f"{wl.name}: {wl.width}x{wl.height} RGBA f32 accumulator, {wl.particles} particles, "
+                               f"{len(wl.flame.xforms)} xforms{' + final' if wl.flame.final_xform else ''}, "
+                               f"{wl.draw_passes} passes/step",
+                   "binned_samples": drawn, "attempted_samples": attempted, "hist_mode": args.hist_mode,
+                   "block_waves": args.waves, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "de_tonemap_ms": round(de_ms, 3),
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

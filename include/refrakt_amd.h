@@ -1,0 +1,289 @@
+/*
+ * refrakt_amd — fractal-flame chaos game, density estimation and tonemap on
+ * AMD Instinct MI355X (gfx950), behind a plain C ABI.
+ *
+ * The entry points replace the reference renderer's hot path
+ * (untrioctium/refrakt):
+ *   flame::set_sim_parameters   src/flame.hpp:77,  src/flame.cpp:105-158
+ *   flame::load_flame           src/flame.hpp:84,  src/flame.cpp:160-226
+ *   flame::warmup               src/flame.hpp:88,  src/flame.cpp:228-281
+ *   flame::draw_to_bins         src/flame.hpp:89,  src/flame.cpp:283-330
+ *   density estimation pass     src/main.cpp:490-515 (+ shaders/density_*.glsl)
+ *   tonemap pass                src/main.cpp:522-535 (+ shaders/tonemap.glsl)
+ *   flame_compiler registry     src/variation_table.hpp:20-59
+ *
+ * Conventions
+ *   - Every call returns RF_OK (0) or a negative RF_ERR_* code; the message of
+ *     the last failure on a context is returned by rf_last_error().  No C++
+ *     exception crosses this boundary.
+ *   - Device memory is owned by the library.  Host arrays passed in are owned
+ *     by the caller and only read or written during the call.
+ *   - Calls on one rf_ctx are serialised by the caller; distinct contexts are
+ *     independent.  Each context owns one HIP stream.
+ *   - Histograms are W*H RGBA float32 bins, row-major, rows stored bottom-up
+ *     exactly like the reference's `bins` SSBO (shaders/flame.glsl:84):
+ *     screen pixel (x, y) lives at bin (H-1-y)*W + x.
+ *   - Images produced by density estimation / tonemap are W*H RGBA float32 in
+ *     the reference's render-target layout (the texture rows of
+ *     render_targets[0..1], src/main.cpp:490-535).
+ */
+#ifndef REFRAKT_AMD_H
+#define REFRAKT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ codes */
+#define RF_OK 0
+#define RF_ERR_ARG -1       /* invalid argument */
+#define RF_ERR_PARSE -2     /* flam3 parse error, unknown attribute or variation */
+#define RF_ERR_HIP -3       /* HIP runtime error */
+#define RF_ERR_STATE -4     /* call not valid in the current state */
+#define RF_ERR_LIMIT -5     /* size beyond a compiled-in limit */
+#define RF_ERR_IO -6        /* file could not be read */
+
+/* --------------------------------------------------------------- limits */
+#define RF_MAX_XFORMS 32          /* non-final xforms per flame */
+#define RF_MAX_VARS_PER_XFORM 16  /* variation terms per xform */
+#define RF_MAX_VAR_PARAMS 8       /* parameters per variation */
+#define RF_PALETTE_SIZE 256
+
+/* ---------------------------------------------------------------- engines */
+/* RF_ENGINE_COMPAT reproduces the reference's pass structure exactly: one
+ * launch per pass, 256-lane workgroups sharing one xform draw, per-slot
+ * jsf32 state, shuffle-row gathers (shaders/flame.glsl:41-90).  With the same
+ * shuffle rows and seeds it gives the reference's bin indices bit for bit on
+ * affine/linear-only flames.
+ * RF_ENGINE_FAST is the MI355X engine: one persistent launch runs all passes,
+ * particles stay in registers, each wave draws one xform per iteration and
+ * particles are exchanged across waves through LDS every iteration. */
+#define RF_ENGINE_COMPAT 0
+#define RF_ENGINE_FAST 1
+
+/* Histogram accumulation strategies of the fast engine. */
+#define RF_HIST_AUTO 0     /* pick from the sampled footprint */
+#define RF_HIST_QUAD 1     /* 4 lanes per RGBA bin: one 16-byte atomic request per sample */
+#define RF_HIST_WAVEAGG 2  /* merge same-bin hits inside a wave before the atomics */
+#define RF_HIST_NAIVE 3    /* 4 separate float atomics per sample (baseline) */
+
+typedef struct rf_ctx rf_ctx;
+typedef struct rf_flame rf_flame;
+typedef struct rf_hist rf_hist;
+
+/* ------------------------------------------------------------ flame model */
+/* One variation term of an xform: the reference keeps these in
+ * std::map<std::string,float> (src/flame.hpp:26-27); rf_flame_create sorts
+ * them by name so they are summed in the same (alphabetical) order. */
+typedef struct rf_var_term {
+  int32_t id;                        /* rf_variation_id(name) */
+  float weight;
+  float params[RF_MAX_VAR_PARAMS];   /* in rf_variation_param_name order */
+} rf_var_term;
+
+/* flame_xform (src/flame.hpp:21-37). affine/post are flam3 `coefs` order
+ * a b c d e f:  x' = a*x + c*y + e,  y' = b*x + d*y + f. */
+typedef struct rf_xform {
+  float affine[6];
+  float post[6];
+  int32_t has_post;
+  int32_t n_vars;
+  rf_var_term vars[RF_MAX_VARS_PER_XFORM];
+  float weight;
+  float color;
+  float color_speed;
+  float rotation_frequency;
+  float opacity;
+} rf_xform;
+
+/* flame (src/flame.hpp:41-60). */
+typedef struct rf_flame_desc {
+  int32_t n_xforms;
+  rf_xform xforms[RF_MAX_XFORMS];
+  int32_t has_final;
+  rf_xform final_xform;
+  float palette[RF_PALETTE_SIZE][4];
+  uint32_t size[2];
+  float center[2];
+  float scale;
+  float rotate;
+  int32_t estimator_min;
+  int32_t estimator_radius;
+  float estimator_curve;
+  float gamma;
+  float vibrancy;
+  float brightness;
+} rf_flame_desc;
+
+/* set_sim_parameters (src/flame.cpp:105-158) plus this library's choices. */
+typedef struct rf_sim_params {
+  uint64_t total_particles;   /* P; a multiple of 256 * temporal_samples */
+  uint32_t temporal_samples;  /* T; a multiple of 32 (animate dispatch, shaders/templates/animate.tpl.glsl) */
+  uint32_t shuffle_count;     /* shuffle rows (compat engine); 0 -> 1024 */
+  uint64_t seed;              /* seeds RNG states, shuffle rows and the row schedule */
+  int32_t device;             /* HIP device ordinal */
+  int32_t engine;             /* RF_ENGINE_* */
+  int32_t hist_mode;          /* RF_HIST_* (fast engine) */
+  int32_t block_waves;        /* fast engine: waves per workgroup (4, 8 or 16); 0 -> 4 */
+} rf_sim_params;
+
+/* Density estimation parameters (src/main.cpp:500-511). */
+typedef struct rf_de_params {
+  int32_t estimator_radius;   /* clamped to 100 like src/main.cpp:502 */
+  int32_t estimator_min;
+  float estimator_curve;
+} rf_de_params;
+
+/* Tonemap parameters (shaders/tonemap.glsl:9-12, src/main.cpp:522-531). */
+typedef struct rf_tonemap_params {
+  float gamma;
+  float brightness;
+  float vibrancy;
+  float scale_constant;       /* k in log10(1 + a*k); the reference passes 1/10^4 */
+} rf_tonemap_params;
+
+typedef struct rf_render_params {
+  uint32_t width, height;     /* accumulator size (image size x supersample) */
+  uint32_t warmup_passes;     /* passes after the first run, not binned */
+  uint32_t draw_passes;       /* binned passes per draw call */
+  uint32_t draw_calls;        /* number of rf_draw_to_bins calls */
+  float tss_width;            /* temporal sample width (src/main.cpp:236) */
+  int32_t density_estimation; /* 0: skip DE (tonemap the histogram directly) */
+  rf_de_params de;            /* ignored unless density_estimation */
+  rf_tonemap_params tm;
+} rf_render_params;
+
+typedef struct rf_stats {
+  uint64_t drawn;             /* binned samples (the reference's atomic counter) */
+  uint64_t iterations;        /* particle iterations, binned or not */
+  double ms_warmup;
+  double ms_draw;
+  double ms_density;
+  double ms_tonemap;
+} rf_stats;
+
+/* ------------------------------------------------------------- context */
+/* ≙ flame::set_sim_parameters + GL context creation. */
+int rf_create(const rf_sim_params* params, rf_ctx** out);
+void rf_destroy(rf_ctx* ctx);
+/* ≙ flame::set_sim_parameters (src/flame.cpp:105-158): reallocates the
+ * simulation buffers; every flame of this context needs a new warmup. */
+int rf_set_sim_parameters(rf_ctx* ctx, uint64_t total_particles,
+                          uint32_t temporal_samples, uint32_t shuffle_count);
+int rf_set_engine(rf_ctx* ctx, int32_t engine, int32_t hist_mode);
+const char* rf_last_error(const rf_ctx* ctx);
+/* Device time (ms, HIP events on the context stream) of the chaos-game
+ * launches of the last rf_draw_to_bins call. */
+double rf_last_draw_ms(const rf_ctx* ctx);
+/* HIP stream of the context (a hipStream_t), for callers that enqueue work
+ * of their own on the same device memory. */
+void* rf_stream(rf_ctx* ctx);
+
+/* ------------------------------------------------------------- flames */
+/* ≙ flame::load_flame (src/flame.cpp:160-226): reads a flam3 XML file. */
+int rf_flame_load(rf_ctx* ctx, const char* path, rf_flame** out);
+/* Same, from XML text in memory. */
+int rf_flame_parse(rf_ctx* ctx, const char* xml, size_t len, rf_flame** out);
+/* Parses flam3 XML into a description without creating a flame. */
+int rf_flame_parse_desc(const char* xml, size_t len, rf_flame_desc* out,
+                        char* err, size_t err_len);
+int rf_flame_create(rf_ctx* ctx, const rf_flame_desc* desc, rf_flame** out);
+/* Replaces the flame's description (e.g. after rotating affines for
+ * animation, src/main.cpp:383-395); the next draw warms up again. */
+int rf_flame_update(rf_flame* flame, const rf_flame_desc* desc);
+int rf_flame_get_desc(const rf_flame* flame, rf_flame_desc* out);
+void rf_flame_destroy(rf_flame* flame);
+/* The reference's flat parameter block (make_shader_buffer_map +
+ * copy_flame_data_to_buffer, src/flame.cpp:33-103): writes at most `cap`
+ * floats and returns the block size in floats (negative on error). */
+int rf_flame_param_block(const rf_flame_desc* desc, float* out, int cap);
+/* Checks a description the way rf_flame_create does (no GPU needed). */
+int rf_flame_validate(const rf_flame_desc* desc, char* err, size_t err_len);
+
+/* ---------------------------------------------------------- histograms */
+int rf_hist_create(rf_ctx* ctx, uint32_t width, uint32_t height, rf_hist** out);
+/* Wraps caller-owned device memory of width*height*16 bytes. */
+int rf_hist_wrap(rf_ctx* ctx, void* device_ptr, uint32_t width, uint32_t height, rf_hist** out);
+void rf_hist_destroy(rf_hist* hist);
+int rf_hist_zero(rf_hist* hist);                    /* ≙ bins.zero_out() */
+int rf_hist_download(rf_hist* hist, float* host);   /* W*H*4 floats */
+int rf_hist_upload(rf_hist* hist, const float* host);
+void* rf_hist_device_ptr(rf_hist* hist);
+
+/* ------------------------------------------------------------ hot path */
+/* ≙ flame::warmup (src/flame.cpp:228-281): animate, one first-run pass
+ * seeded from the Hammersley points, then num_passes passes; nothing binned. */
+int rf_warmup(rf_ctx* ctx, rf_flame* flame, uint32_t num_passes, float tss_width);
+/* ≙ flame::draw_to_bins (src/flame.cpp:283-330): num_iter binned passes;
+ * *drawn receives the number of binned samples (may be NULL). */
+int rf_draw_to_bins(rf_ctx* ctx, rf_flame* flame, rf_hist* bins,
+                    uint32_t num_iter, uint64_t* drawn);
+/* Compat-engine variants with an explicit shuffle-row schedule: rows holds
+ * (in_row, out_row) pairs, one per dispatched pass (warmup: num_passes + 1
+ * pairs, the first for the first run).  The reference draws these with an
+ * unseeded std::mt19937 (src/flame.cpp:231-233, 285-287). */
+int rf_warmup_rows(rf_ctx* ctx, rf_flame* flame, uint32_t num_passes,
+                   float tss_width, const uint32_t* rows);
+int rf_draw_to_bins_rows(rf_ctx* ctx, rf_flame* flame, rf_hist* bins,
+                         uint32_t num_iter, const uint32_t* rows, uint64_t* drawn);
+/* Density estimation into `out` (device memory of W*H*16 bytes, or NULL to
+ * use the histogram's own image buffer), ≙ src/main.cpp:490-515. */
+int rf_density_estimate(rf_ctx* ctx, rf_hist* bins, const rf_de_params* de, void* out_device);
+/* Tonemap `in_device` (W*H RGBA) into `out_device`, ≙ src/main.cpp:522-535. */
+int rf_tonemap(rf_ctx* ctx, const void* in_device, void* out_device,
+               uint32_t width, uint32_t height, const rf_tonemap_params* tm);
+/* Fused density estimation + tonemap of a histogram into out_device. */
+int rf_density_tonemap(rf_ctx* ctx, rf_hist* bins, const rf_de_params* de,
+                       const rf_tonemap_params* tm, void* out_device);
+/* The whole path: warmup, draw_calls x draw_to_bins, DE, tonemap.  hist_out
+ * and image_out are host arrays of W*H*4 floats (either may be NULL). */
+int rf_render(rf_ctx* ctx, rf_flame* flame, const rf_render_params* params,
+              float* hist_out, float* image_out, rf_stats* stats);
+
+/* ---------------------------------------------------- state inspection */
+/* Particle buffer (P x (x, y, color, 0)) after the last pass. */
+int rf_particles_download(rf_ctx* ctx, rf_flame* flame, float* host);
+/* Compat engine: jsf32 state per slot (P x 4 u32), shuffle rows
+ * (shuffle_count x P/T u32) and Hammersley points (P/T x 4 floats). */
+int rf_rng_download(rf_ctx* ctx, uint32_t* host);
+int rf_shuffle_download(rf_ctx* ctx, uint32_t* host);
+int rf_sample_points_download(rf_ctx* ctx, float* host);
+/* Fast engine: the 256 LDS permutation rows (256 x 64*block_waves u16). */
+int rf_fast_perms_download(rf_ctx* ctx, uint16_t* host);
+/* Screen-space affine used by draw_to_bins (src/flame.cpp:289-296). */
+int rf_screen_affine(const rf_flame_desc* desc, uint32_t width, uint32_t height, float out[6]);
+
+/* ------------------------------------------------ variation registry */
+/* ≙ flame_compiler (src/variation_table.hpp:20-59). */
+int rf_variation_count(void);
+const char* rf_variation_name(int id);
+int rf_variation_id(const char* name);               /* -1 if unknown */
+int rf_variation_num_params(int id);
+const char* rf_variation_param_name(int id, int k);
+/* Bit 0: no_weight_mul, bit 1: pre_xform, bit 8: consumes random numbers. */
+int rf_variation_flags(int id);
+
+/* ------------------------------------------------- diagnostics / tests */
+/* Applies one xform (affine, variations, post, colour blend) to n points on
+ * the GPU.  pts: n x (x, y, c); rands: n x 16 uniforms consumed in order by
+ * RAND(); out: n x (x, y, c, opacity).  Host arrays. */
+int rf_eval_xform(rf_ctx* ctx, const rf_xform* xf, const float* pts,
+                  const float* rands, float* out, uint32_t n);
+/* Measures the L2 float-atomic ceiling: `ops` RGBA samples scattered over a
+ * footprint of `footprint_bins` bins with the given strategy; returns the
+ * kernel time in ms through *ms. */
+int rf_atomic_probe(rf_ctx* ctx, int32_t hist_mode, uint64_t footprint_bins,
+                    uint64_t samples, double* ms);
+
+/* sizeof of the ABI structs (0 rf_var_term, 1 rf_xform, 2 rf_flame_desc,
+ * 3 rf_sim_params, 4 rf_de_params, 5 rf_tonemap_params, 6 rf_render_params,
+ * 7 rf_stats), for bindings that check their layouts. */
+size_t rf_struct_size(int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* REFRAKT_AMD_H */

@@ -1,0 +1,278 @@
+// Device-side flame representation, random number generators and the xform
+// interpreter shared by the compat and fast chaos-game engines.
+//
+// The reference compiles one GLSL `dispatch(v, xform)` per flame
+// (src/variation_table.cpp:217-265).  Here a flame is data: a `Block` in
+// device memory, read with wave-uniform addresses (scalar loads), and each
+// wave runs one xform at a time so the variation `switch` is uniform.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rfd {
+
+constexpr int MAX_XF = 32;          // non-final xforms per flame
+constexpr int FINAL = MAX_XF;       // slot of the final xform in Block::xf
+constexpr int MAX_TERMS = 256;      // variation terms per flame (all xforms)
+constexpr int MAX_PRM = 8;
+
+constexpr float PI = 3.141592653589793f;   // shaders/include/math.glsl:1
+constexpr float PI_2 = PI / 2.0f;
+constexpr float EPS = 1e-10f;              // shaders/include/math.glsl:4
+constexpr float M_2_PI_F = 0.636619772367581343f;
+constexpr float LOG10E = 0.434294481903251827651128918916f;  // math.glsl log10
+
+// precalc bits (must match gen_variations.py)
+constexpr uint32_t PC_R = 1u << 0, PC_RSQ = 1u << 1, PC_A = 1u << 2, PC_PHI = 1u << 3,
+                   PC_SINA = 1u << 4, PC_COSA = 1u << 5, PC_SINR = 1u << 6, PC_COSR = 1u << 7;
+// term flags (variations.yaml flags)
+constexpr uint32_t TF_NO_WEIGHT_MUL = 1u, TF_PRE_XFORM = 2u;
+// xform flags
+constexpr uint32_t XF_HAS_POST = 1u;
+
+struct Term {            // 48 bytes
+  uint32_t var;          // variation id
+  uint32_t flags;        // TF_*
+  float weight;
+  float prm[MAX_PRM];
+  uint32_t pad;
+};
+
+struct Xform {           // 80 bytes
+  float aff[6];          // a b c d e f ; index = col*2 + row (c00 c01 c10 c11 c20 c21)
+  float post[6];
+  float color, color_speed, opacity;
+  uint32_t flags;
+  uint32_t precalc;      // union of the regular terms' precalc needs
+  uint32_t term_begin;   // first term in Block::terms
+  uint32_t n_pre;        // leading pre_xform terms
+  uint32_t n_terms;      // all terms
+};
+
+struct Block {
+  uint32_t n_xforms;     // non-final xforms (>= 1)
+  uint32_t has_final;
+  uint32_t n_terms;
+  uint32_t pad;
+  float cdf[MAX_XF];     // running sum of normalised weights, in xform order
+  Xform xf[MAX_XF + 1];
+  Term terms[MAX_TERMS];
+};
+
+// ---------------------------------------------------------------- helpers
+__host__ __device__ __forceinline__ float rf_mod2(float x, float y) {
+  return x - y * truncf(x / y);  // shaders/include/math.glsl mod2
+}
+__host__ __device__ __forceinline__ bool rf_badval(float x) {
+  return (x != x) || (x > 1e10f) || (x < -1e10f);  // shaders/include/math.glsl badval
+}
+
+// sin/cos on [0, 1] by their Taylor series in Horner form (truncation error
+// < 2e-10), written with explicit fmaf so the CPU oracle and the GPU produce
+// identical bits.  Used for the first-run jitter angle, which is sqrt(U[0,1]).
+__host__ __device__ __forceinline__ void rf_sincos01(float q, float* s, float* c) {
+  float z = q * q;
+  float ps = -2.5052108385441720e-08f;
+  ps = fmaf(ps, z, 2.7557319223985893e-06f);
+  ps = fmaf(ps, z, -1.9841269841269841e-04f);
+  ps = fmaf(ps, z, 8.3333333333333332e-03f);
+  ps = fmaf(ps, z, -1.6666666666666666e-01f);
+  ps = fmaf(ps, z, 1.0f);
+  *s = q * ps;
+  float pc = 2.0876756987868099e-09f;
+  pc = fmaf(pc, z, -2.7557319223985888e-07f);
+  pc = fmaf(pc, z, 2.4801587301587302e-05f);
+  pc = fmaf(pc, z, -1.3888888888888889e-03f);
+  pc = fmaf(pc, z, 4.1666666666666664e-02f);
+  pc = fmaf(pc, z, -0.5f);
+  *c = fmaf(pc, z, 1.0f);
+}
+
+// Palette index min(255, uint(ceil(c * 255))) (shaders/flame.glsl:84), with
+// negative and NaN colours mapped to 0 (undefined in GLSL).
+__host__ __device__ __forceinline__ uint32_t rf_pal_index(float c) {
+  float t = ceilf(c * 255.0f);
+  if (!(t > 0.0f)) return 0u;
+  if (t >= 255.0f) return 255u;
+  return (uint32_t)t;
+}
+
+// GLSL mix(x, y, a) = x*(1-a) + y*a, evaluated without contraction.
+__host__ __device__ __forceinline__ float rf_mix(float x, float y, float a) {
+  float t0 = x * (1.0f - a);
+  float t1 = y * a;
+  return t0 + t1;
+}
+
+// ------------------------------------------------------------------ RNGs
+// Bob Jenkins' small fast PRNG (JSF32), the generator of the reference
+// (src/util.hpp:75-96, shaders/include/random.glsl:29-41).  The GPU side of
+// the reference returns the new `a`; the CPU seeding returns `d`.
+struct Jsf32 {
+  uint32_t a, b, c, d;
+  __host__ __device__ __forceinline__ static uint32_t rot(uint32_t x, int k) {
+    return (x << k) | (x >> (32 - k));
+  }
+  __host__ __device__ __forceinline__ void step() {
+    uint32_t e = a - rot(b, 27);
+    a = b ^ rot(c, 17);
+    b = c + d;
+    c = d + e;
+    d = e + a;
+  }
+  __host__ __device__ __forceinline__ uint32_t next_a() { step(); return a; }  // random.glsl:30-37
+  __host__ __device__ __forceinline__ uint32_t next_d() { step(); return d; }  // util.hpp:81-88
+  // randf(): clamp(float(u) / 4294967295.0f, 0, 1); the literal rounds to 2^32
+  // so this is float(u) * 2^-32 exactly (random.glsl:39-41).
+  __host__ __device__ __forceinline__ float randf() { return (float)next_a() * 0x1p-32f; }
+  // jsf32::warmup_ctx (src/util.hpp:90-95)
+  __host__ __device__ static Jsf32 seeded(uint32_t seed) {
+    Jsf32 s{0xf1ea5eedu, seed, seed, seed};
+    for (int i = 0; i < 20; ++i) s.step();
+    return s;
+  }
+};
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11).  Counter-based: the fast engine derives every
+// lane's and wave's stream from (seed, particle/wave id, launch) with it.
+struct Philox {
+  __host__ __device__ static inline void round(uint32_t ctr[4], const uint32_t key[2]) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * ctr[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * ctr[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ ctr[1] ^ key[0];
+    uint32_t n1 = lo1;
+    uint32_t n2 = hi0 ^ ctr[3] ^ key[1];
+    uint32_t n3 = lo0;
+    ctr[0] = n0; ctr[1] = n1; ctr[2] = n2; ctr[3] = n3;
+  }
+  __host__ __device__ static inline void gen(const uint32_t in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c[4] = {in[0], in[1], in[2], in[3]};
+    uint32_t k[2] = {key_in[0], key_in[1]};
+    for (int i = 0; i < 10; ++i) {
+      if (i) { k[0] += 0x9E3779B9u; k[1] += 0xBB67AE85u; }
+      round(c, k);
+    }
+    out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = c[3];
+  }
+};
+
+// A JSF32 state seeded from a Philox block, warmed like warmup_ctx.
+__host__ __device__ inline Jsf32 jsf_from_philox(uint64_t seed, uint32_t id, uint32_t launch, uint32_t stream) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t ctr[4] = {id, launch, stream, 0x5EEDu};
+  uint32_t o[4];
+  Philox::gen(ctr, key, o);
+  Jsf32 s{0xf1ea5eedu ^ o[0], o[1], o[2], o[3] | 1u};
+  for (int i = 0; i < 8; ++i) s.step();
+  return s;
+}
+
+// Random source backed by a table (diagnostic kernel and tests).
+struct TableRng {
+  const float* p;
+  int k;
+  __device__ __forceinline__ float randf() { return p[k++]; }
+};
+
+// ----------------------------------------------------------- interpreter
+struct Pre {
+  float r, rsq, a, phi, sina, cosa, sinr, cosr;
+};
+
+template <class Rng>
+__device__ __forceinline__ void eval_var(uint32_t id, float x, float y, const Pre& pc, float w,
+                                         const float* __restrict__ A, const float* __restrict__ P,
+                                         Rng& rng, float& ox, float& oy) {
+  const float r = pc.r, rsq = pc.rsq, a = pc.a, phi = pc.phi;
+  const float sina = pc.sina, cosa = pc.cosa, sinr = pc.sinr, cosr = pc.cosr;
+  (void)r; (void)rsq; (void)a; (void)phi; (void)sina; (void)cosa; (void)sinr; (void)cosr;
+  (void)w; (void)A; (void)P;
+  switch (id) {
+#include "gen/variation_cases.inc"
+    default: ox = 0.0f; oy = 0.0f; break;
+  }
+}
+
+// Result of one xform application: the new point, colour and opacity
+// (dispatch() in the generated GLSL returns vec4(result, color, opacity)).
+struct XOut { float x, y, c, w; };
+
+// Applies xform `xi` of `blk` to (x0, y0, c0).  If `first` the incoming
+// colour is replaced by a random draw made after the variations' draws, as
+// the generated code does on the first run (variation_table.cpp:241-243).
+// Evaluation order follows variation_table.cpp:97-165: fma affine, pre_xform
+// terms added into v, precalcs, the regular terms summed in alphabetical order
+// (separate multiply and add), then the fma post-affine.
+template <class Rng>
+__device__ __forceinline__ XOut apply_xform(const Block* __restrict__ blk, int xi, float x0, float y0,
+                                            float c0, bool first, Rng& rng) {
+  const Xform* __restrict__ xf = &blk->xf[xi];
+  float A[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) A[k] = xf->aff[k];
+  float x = fmaf(A[0], x0, fmaf(A[2], y0, A[4]));
+  float y = fmaf(A[1], x0, fmaf(A[3], y0, A[5]));
+  const uint32_t tb = xf->term_begin, npre = xf->n_pre, nt = xf->n_terms;
+  const Term* __restrict__ T = blk->terms + tb;
+  Pre pc;
+  pc.r = pc.rsq = pc.a = pc.phi = pc.sina = pc.cosa = pc.sinr = pc.cosr = 0.0f;
+  for (uint32_t k = 0; k < npre; ++k) {
+    float P[MAX_PRM];
+#pragma unroll
+    for (int j = 0; j < MAX_PRM; ++j) P[j] = T[k].prm[j];
+    const float w = T[k].weight;
+    float ox, oy;
+    eval_var(T[k].var, x, y, pc, w, A, P, rng, ox, oy);
+    if (!(T[k].flags & TF_NO_WEIGHT_MUL)) { ox = w * ox; oy = w * oy; }
+    x = x + ox;
+    y = y + oy;
+  }
+  const uint32_t m = xf->precalc;
+  if (m & (PC_RSQ | PC_R)) pc.rsq = x * x + y * y;
+  if (m & PC_R) pc.r = sqrtf(pc.rsq);
+  if (m & PC_A) pc.a = atan2f(x, y);
+  if (m & PC_PHI) pc.phi = atan2f(y, x);
+  if (m & PC_SINA) pc.sina = x / pc.r;
+  if (m & PC_COSA) pc.cosa = y / pc.r;
+  if (m & PC_SINR) pc.sinr = sinf(pc.r);
+  if (m & PC_COSR) pc.cosr = cosf(pc.r);
+  float rx = 0.0f, ry = 0.0f;
+  for (uint32_t k = npre; k < nt; ++k) {
+    float P[MAX_PRM];
+#pragma unroll
+    for (int j = 0; j < MAX_PRM; ++j) P[j] = T[k].prm[j];
+    const float w = T[k].weight;
+    float ox, oy;
+    eval_var(T[k].var, x, y, pc, w, A, P, rng, ox, oy);
+    if (!(T[k].flags & TF_NO_WEIGHT_MUL)) { ox = w * ox; oy = w * oy; }
+    if (k == npre) { rx = ox; ry = oy; }
+    else { rx = rx + ox; ry = ry + oy; }
+  }
+  if (xf->flags & XF_HAS_POST) {
+    const float q0 = xf->post[0], q1 = xf->post[1], q2 = xf->post[2];
+    const float q3 = xf->post[3], q4 = xf->post[4], q5 = xf->post[5];
+    float px = fmaf(q0, rx, fmaf(q2, ry, q4));
+    float py = fmaf(q1, rx, fmaf(q3, ry, q5));
+    rx = px; ry = py;
+  }
+  float cin = first ? rng.randf() : c0;
+  XOut o;
+  o.x = rx; o.y = ry;
+  o.c = rf_mix(cin, xf->color, xf->color_speed);
+  o.w = xf->opacity;
+  return o;
+}
+
+// get_xform_id (shaders/templates/xform_select.tpl.glsl:1-14): the first i
+// whose running weight sum reaches u; the last xform is unconditional.
+__device__ __forceinline__ int select_xform(const Block* __restrict__ blk, float u) {
+  const int n = (int)blk->n_xforms;
+  int i = 0;
+  while (i < n - 1 && !(blk->cdf[i] >= u)) ++i;
+  return i;
+}
+
+}  // namespace rfd

@@ -1,0 +1,865 @@
+// Host library: flame model, simulation state and the C ABI entry points.
+//
+// The orchestration mirrors src/flame.cpp (set_sim_parameters, warmup,
+// draw_to_bins) and the inline DE/tonemap passes of src/main.cpp:490-535;
+// the device work is done by the kernels of rf_kernels.hip on one HIP stream
+// per context.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "rf_host.hpp"
+#include "rf_internal.hpp"
+
+#include "gen/variation_table.inc"
+
+using rfd::Block;
+
+namespace rfh {
+
+hipError_t DevBuf::alloc(size_t n) {
+  if (p && bytes == n) return hipSuccess;
+  release();
+  if (n == 0) return hipSuccess;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) { p = nullptr; bytes = 0; return e; }
+  bytes = n;
+  return hipSuccess;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+const rf_var_info* var_info(int id) {
+  if (id < 0 || id >= RF_GEN_NUM_VARIATIONS) return nullptr;
+  return &k_var_table[id];
+}
+
+namespace {
+struct ParamIndex {
+  std::unordered_map<std::string, std::pair<int, int>> params;
+  std::unordered_map<std::string, int> vars;
+  ParamIndex() {
+    for (int v = 0; v < RF_GEN_NUM_VARIATIONS; ++v) {
+      vars[k_var_table[v].name] = v;
+      for (int k = 0; k < k_var_table[v].n_params; ++k) params[k_var_table[v].params[k]] = {v, k};
+    }
+  }
+};
+const ParamIndex& pindex() {
+  static ParamIndex idx;
+  return idx;
+}
+
+// splitmix64 (Steele, Lea, Flood: "Fast splittable pseudorandom number
+// generators", OOPSLA'14); drives the shuffle tables and row schedule.
+uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+uint32_t bounded(uint64_t& s, uint32_t n) {
+  return (uint32_t)(((uint64_t)(uint32_t)(splitmix64(s) >> 32) * n) >> 32);
+}
+}  // namespace
+
+bool param_lookup(const char* name, int* var, int* k) {
+  auto& idx = pindex();
+  auto it = idx.params.find(name);
+  if (it == idx.params.end()) return false;
+  *var = it->second.first;
+  *k = it->second.second;
+  return true;
+}
+
+// Random permutation of [0, n) (Fisher-Yates), the build's stand-in for the
+// reference's std::shuffle rows (src/shuffle_buffers.cpp:9-21).
+void shuffle_row(uint32_t* row, uint32_t n, uint64_t seed) {
+  uint64_t s = seed;
+  for (uint32_t i = 0; i < n; ++i) row[i] = i;
+  for (uint32_t i = n - 1; i > 0; --i) {
+    uint32_t j = bounded(s, i + 1);
+    std::swap(row[i], row[j]);
+  }
+}
+
+static bool name_less(int a, int b) { return strcmp(k_var_table[a].name, k_var_table[b].name) < 0; }
+
+bool build_block(const rf_flame_desc& d, Block& b, std::string& err) {
+  memset(&b, 0, sizeof(b));
+  if (d.n_xforms < 1 || d.n_xforms > rfd::MAX_XF) { err = "n_xforms out of range"; return false; }
+  b.n_xforms = (uint32_t)d.n_xforms;
+  b.has_final = d.has_final ? 1u : 0u;
+  // copy_flame_data_to_buffer (src/flame.cpp:78-82): weights normalised by
+  // the float sum over the non-final xforms; get_xform_id sums them in order.
+  float total = 0.0f;
+  for (int i = 0; i < d.n_xforms; ++i) total += d.xforms[i].weight;
+  if (!(total > 0.0f)) { err = "xform weights must sum to a positive value"; return false; }
+  float sum = 0.0f;
+  for (int i = 0; i < d.n_xforms; ++i) {
+    const float wn = d.xforms[i].weight / total;
+    sum = (i == 0) ? wn : sum + wn;
+    b.cdf[i] = sum;
+  }
+  uint32_t nt = 0;
+  auto put = [&](const rf_xform& x, rfd::Xform& X) -> bool {
+    for (int k = 0; k < 6; ++k) { X.aff[k] = x.affine[k]; X.post[k] = x.post[k]; }
+    X.flags = x.has_post ? rfd::XF_HAS_POST : 0u;
+    X.color = x.color;
+    X.color_speed = x.color_speed;
+    X.opacity = x.opacity;
+    if (x.n_vars < 0 || x.n_vars > RF_MAX_VARS_PER_XFORM) { err = "n_vars out of range"; return false; }
+    // std::map<std::string,float> semantics: one entry per name (last wins),
+    // iterated in name order (src/flame.hpp:26).
+    std::vector<int> order;
+    for (int t = 0; t < x.n_vars; ++t) {
+      const int id = x.vars[t].id;
+      if (!var_info(id)) { err = "unknown variation id " + std::to_string(id); return false; }
+      bool dup = false;
+      for (int q = t + 1; q < x.n_vars; ++q) if (x.vars[q].id == id) dup = true;
+      if (!dup) order.push_back(t);
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return name_less(x.vars[a].id, x.vars[c].id); });
+    std::vector<int> pre, reg;
+    for (int t : order) ((k_var_table[x.vars[t].id].flags & rfd::TF_PRE_XFORM) ? pre : reg).push_back(t);
+    X.term_begin = nt;
+    X.n_pre = (uint32_t)pre.size();
+    X.n_terms = (uint32_t)(pre.size() + reg.size());
+    X.precalc = 0;
+    if (nt + X.n_terms > (uint32_t)rfd::MAX_TERMS) { err = "too many variation terms in the flame"; return false; }
+    std::vector<int> all(pre);
+    all.insert(all.end(), reg.begin(), reg.end());
+    for (int t : all) {
+      const rf_var_term& v = x.vars[t];
+      rfd::Term& T = b.terms[nt++];
+      T.var = (uint32_t)v.id;
+      T.flags = k_var_table[v.id].flags & (rfd::TF_NO_WEIGHT_MUL | rfd::TF_PRE_XFORM);
+      T.weight = v.weight;
+      for (int k = 0; k < rfd::MAX_PRM; ++k) T.prm[k] = (k < k_var_table[v.id].n_params) ? v.params[k] : 0.0f;
+      if (!(T.flags & rfd::TF_PRE_XFORM)) X.precalc |= k_var_table[v.id].precalc;
+    }
+    return true;
+  };
+  for (int i = 0; i < d.n_xforms; ++i) if (!put(d.xforms[i], b.xf[i])) return false;
+  if (d.has_final && !put(d.final_xform, b.xf[rfd::FINAL])) return false;
+  b.n_terms = nt;
+  return true;
+}
+
+std::vector<float> reference_param_block(const rf_flame_desc& d) {
+  std::vector<float> out;
+  float normal_weight = 0.0f;
+  for (int i = 0; i < d.n_xforms; ++i) normal_weight += d.xforms[i].weight;
+  auto push = [&](const rf_xform& x) {
+    out.push_back(x.weight / normal_weight);
+    for (int a = 0; a < 6; ++a) out.push_back(x.affine[a]);
+    if (x.has_post) for (int a = 0; a < 6; ++a) out.push_back(x.post[a]);
+    std::vector<std::pair<std::string, float>> vars, prms;
+    for (int t = 0; t < x.n_vars; ++t) {
+      const rf_var_info* vi = var_info(x.vars[t].id);
+      if (!vi) continue;
+      vars.emplace_back(vi->name, x.vars[t].weight);
+      for (int k = 0; k < vi->n_params; ++k) prms.emplace_back(vi->params[k], x.vars[t].params[k]);
+    }
+    auto by_name = [](const std::pair<std::string, float>& a, const std::pair<std::string, float>& b) { return a.first < b.first; };
+    std::stable_sort(vars.begin(), vars.end(), by_name);
+    std::stable_sort(prms.begin(), prms.end(), by_name);
+    for (auto& v : vars) out.push_back(v.second);
+    for (auto& v : prms) out.push_back(v.second);
+    out.push_back(x.color);
+    out.push_back(x.color_speed);
+    out.push_back(x.opacity);
+    out.push_back(x.rotation_frequency);
+  };
+  for (int i = 0; i < d.n_xforms; ++i) push(d.xforms[i]);
+  if (d.has_final) push(d.final_xform);
+  return out;
+}
+
+void screen_affine(const rf_flame_desc& d, uint32_t w, uint32_t h, float out[6]) {
+  float b[6] = {1, 0, 0, 1, 0, 0};
+  auto translate = [](float a[6], float t0, float t1) {
+    const float r4 = a[0] * t0 + a[2] * t1 + a[4];
+    const float r5 = a[1] * t0 + a[3] * t1 + a[5];
+    a[4] = r4;
+    a[5] = r5;
+  };
+  translate(b, (float)w / 2.0f, (float)h / 2.0f);
+  const float s = d.scale * (float)h / (float)d.size[1];
+  for (int k = 0; k < 4; ++k) b[k] = b[k] * s;
+  const float rad = 0.01745329251f * d.rotate;
+  const float sino = sinf(rad), coso = cosf(rad);
+  const float r0 = b[0] * coso + b[2] * sino;
+  const float r1 = b[1] * coso + b[3] * sino;
+  const float r2 = b[2] * coso - b[0] * sino;
+  const float r3 = b[3] * coso - b[1] * sino;
+  b[0] = r0; b[1] = r1; b[2] = r2; b[3] = r3;
+  translate(b, -d.center[0], -d.center[1]);
+  for (int k = 0; k < 6; ++k) out[k] = b[k];
+}
+
+}  // namespace rfh
+
+using namespace rfh;
+
+// ---------------------------------------------------------------- helpers
+static int set_err(rf_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      return set_err((ctx), RF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+static int validate_sim(rf_ctx* ctx, uint64_t P, uint32_t T) {
+  if (T == 0) return set_err(ctx, RF_ERR_ARG, "temporal_samples must be positive");
+  if (P == 0 || P % (256ull * T) != 0)
+    return set_err(ctx, RF_ERR_ARG, "total_particles must be a positive multiple of 256 * temporal_samples");
+  if (P / T > 0xffffffffull) return set_err(ctx, RF_ERR_LIMIT, "particles per temporal sample exceed 2^32");
+  return RF_OK;
+}
+
+static int ensure_compat(rf_ctx* ctx) {
+  if (ctx->compat_ready) return RF_OK;
+  const uint64_t P = ctx->sp.total_particles;
+  const uint64_t ppt = ctx->ppt;
+  HIPCHK(ctx, ctx->swap.alloc(P * 16));
+  HIPCHK(ctx, ctx->rs.alloc(P * 16));
+  HIPCHK(ctx, rfk::launch_init_rs((uint4*)ctx->rs.p, P, ctx->stream));
+  // shuffle rows (set_sim_parameters, src/flame.cpp:114-130)
+  const uint32_t R = ctx->shuffle_count;
+  std::vector<uint32_t> rows((size_t)R * ppt);
+  for (uint32_t r = 0; r < R; ++r)
+    shuffle_row(rows.data() + (size_t)r * ppt, (uint32_t)ppt, ctx->sp.seed ^ (0xA5A5A5A5ull * (r + 1)));
+  HIPCHK(ctx, ctx->shuf.alloc(rows.size() * 4));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->shuf.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  // Hammersley sample points (src/hammersley.cpp:29-48)
+  std::vector<float> pts(ppt * 4, 0.0f);
+  {
+    const uint32_t cnt = (uint32_t)ppt;
+    uint32_t mx = cnt;
+    if (cnt % 2 != 0) { uint32_t v = cnt - 1; v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16; mx = v + 1; }
+    const float inv_max = 1.0f / (float)mx;
+    uint32_t r = 0, v = mx;
+    while (v >>= 1) ++r;
+    for (uint32_t i = 0; i < cnt; ++i) {
+      uint32_t n = i;
+      n = (n << 16) | (n >> 16);
+      n = ((n & 0x00ff00ffu) << 8) | ((n & 0xff00ff00u) >> 8);
+      n = ((n & 0x0f0f0f0fu) << 4) | ((n & 0xf0f0f0f0u) >> 4);
+      n = ((n & 0x33333333u) << 2) | ((n & 0xccccccccu) >> 2);
+      n = ((n & 0x55555555u) << 1) | ((n & 0xaaaaaaaau) >> 1);
+      const uint32_t fl = r ? (n >> (32 - r)) : 0u;
+      pts[4 * (size_t)i + 0] = ((float)i * inv_max) * 2.0f - 1.0f;
+      pts[4 * (size_t)i + 1] = ((float)fl * inv_max) * 2.0f - 1.0f;
+    }
+  }
+  HIPCHK(ctx, ctx->samples.alloc(pts.size() * 4));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->samples.p, pts.data(), pts.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->compat_ready = true;
+  return RF_OK;
+}
+
+static int fast_waves(const rf_ctx* ctx) {
+  const int w = ctx->sp.block_waves;
+  return (w == 8 || w == 16) ? w : 4;
+}
+
+static int ensure_perms(rf_ctx* ctx) {
+  const int waves = fast_waves(ctx);
+  if (ctx->perms.p && ctx->perm_waves == waves) return RF_OK;
+  const uint32_t N = 64u * (uint32_t)waves;
+  std::vector<uint16_t> tab((size_t)rfk::NPERM * N);
+  std::vector<uint32_t> row(N);
+  for (int r = 0; r < rfk::NPERM; ++r) {
+    shuffle_row(row.data(), N, ctx->sp.seed ^ 0x5EED5EED00000000ull ^ (uint64_t)(r + 1) * 0x2545F4914F6CDD1Dull);
+    for (uint32_t i = 0; i < N; ++i) tab[(size_t)r * N + i] = (uint16_t)row[i];
+  }
+  HIPCHK(ctx, ctx->perms.alloc(tab.size() * 2));
+  HIPCHK(ctx, hipMemcpy(ctx->perms.p, tab.data(), tab.size() * 2, hipMemcpyHostToDevice));
+  ctx->perm_waves = waves;
+  return RF_OK;
+}
+
+static void next_rows(rf_ctx* ctx, uint32_t n_pairs, std::vector<uint32_t>& rows) {
+  rows.resize(2 * (size_t)n_pairs);
+  for (auto& r : rows) r = bounded(ctx->row_state, ctx->shuffle_count);
+}
+
+static int flame_upload(rf_flame* f, float tss_width) {
+  rf_ctx* ctx = f->ctx;
+  Block b;
+  std::string err;
+  if (!build_block(f->desc, b, err)) return set_err(ctx, RF_ERR_ARG, err);
+  bool animated = false;
+  for (int i = 0; i < f->desc.n_xforms; ++i) if (f->desc.xforms[i].rotation_frequency != 0.0f) animated = true;
+  if (f->desc.has_final && f->desc.final_xform.rotation_frequency != 0.0f) animated = true;
+  const uint32_t T = ctx->sp.temporal_samples;
+  f->animated = animated;
+  f->n_blocks = animated ? T : 1u;
+  HIPCHK(ctx, f->blocks.alloc(sizeof(Block) * (animated ? (size_t)T + 1 : 1)));
+  const size_t pristine = animated ? T : 0;
+  HIPCHK(ctx, hipMemcpyAsync((Block*)f->blocks.p + pristine, &b, sizeof(Block), hipMemcpyHostToDevice, ctx->stream));
+  if (animated) {
+    float rot[rfd::MAX_XF + 1] = {0};
+    for (int i = 0; i < f->desc.n_xforms; ++i) rot[i] = f->desc.xforms[i].rotation_frequency;
+    if (f->desc.has_final) rot[f->desc.n_xforms] = f->desc.final_xform.rotation_frequency;
+    HIPCHK(ctx, f->rot.alloc(sizeof(rot)));
+    HIPCHK(ctx, hipMemcpyAsync(f->rot.p, rot, sizeof(rot), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, rfk::launch_animate((Block*)f->blocks.p, T, tss_width, (const float*)f->rot.p, ctx->stream));
+  }
+  HIPCHK(ctx, f->palette.alloc(256 * 16));
+  HIPCHK(ctx, hipMemcpyAsync(f->palette.p, f->desc.palette, 256 * 16, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  f->needs_update = false;
+  return RF_OK;
+}
+
+static int check_desc(rf_ctx* ctx, const rf_flame_desc* d) {
+  Block b;
+  std::string err;
+  if (!build_block(*d, b, err)) return set_err(ctx, RF_ERR_ARG, err);
+  return RF_OK;
+}
+
+static inline rfk::CompatArgs compat_args(rf_ctx* ctx, rf_flame* f) {
+  rfk::CompatArgs a{};
+  a.rs = (uint4*)ctx->rs.p;
+  a.shuf = (const uint32_t*)ctx->shuf.p;
+  a.blocks = (const Block*)f->blocks.p;
+  a.block_stride = f->animated ? 1u : 0u;
+  a.palette = (const float4*)f->palette.p;
+  a.counter = (unsigned long long*)ctx->counter.p;
+  a.ppt = (uint32_t)ctx->ppt;
+  a.T = ctx->sp.temporal_samples;
+  return a;
+}
+
+static inline rfk::FastArgs fast_args(rf_ctx* ctx, rf_flame* f) {
+  rfk::FastArgs a{};
+  a.pos = (float4*)f->local.p;
+  a.blocks = (const Block*)f->blocks.p;
+  a.block_stride = f->animated ? 1u : 0u;
+  a.palette = (const float4*)f->palette.p;
+  a.counter = (unsigned long long*)ctx->counter.p;
+  a.perms = (const uint16_t*)ctx->perms.p;
+  a.seed = ctx->sp.seed;
+  a.ppt = (uint32_t)ctx->ppt;
+  return a;
+}
+
+static int do_warmup(rf_ctx* ctx, rf_flame* f, uint32_t num_passes, float tss, const uint32_t* rows_in) {
+  if (!ctx || !f || f->ctx != ctx) return set_err(ctx, RF_ERR_ARG, "bad context or flame");
+  const uint64_t P = ctx->sp.total_particles;
+  if (f->generation != ctx->generation) { f->local.release(); f->have_particles = false; f->generation = ctx->generation; }
+  HIPCHK(ctx, f->local.alloc(P * 16));
+  int rc = flame_upload(f, tss);
+  if (rc) return rc;
+  if (ctx->sp.engine == RF_ENGINE_COMPAT) {
+    if ((rc = ensure_compat(ctx))) return rc;
+    std::vector<uint32_t> rows;
+    if (rows_in) rows.assign(rows_in, rows_in + 2 * ((size_t)num_passes + 1));
+    else next_rows(ctx, num_passes + 1, rows);
+    for (auto r : rows) if (r >= ctx->shuffle_count) return set_err(ctx, RF_ERR_ARG, "shuffle row out of range");
+    rfk::CompatArgs a = compat_args(ctx, f);
+    a.pos_in = (const float4*)ctx->samples.p;
+    a.pos_out = (float4*)f->local.p;
+    a.random_read = 1; a.random_write = 1; a.first_run = 1; a.do_draw = 0;
+    a.in_row = rows[0]; a.out_row = rows[1];
+    HIPCHK(ctx, rfk::launch_compat_pass(a, ctx->sp.hist_mode, ctx->stream));
+    void* names[2] = {f->local.p, ctx->swap.p};
+    a.first_run = 0;
+    for (uint32_t i = 0; i < num_passes; ++i) {
+      a.pos_in = (const float4*)names[i % 2];
+      a.pos_out = (float4*)names[(i + 1) % 2];
+      a.in_row = rows[2 * (i + 1)]; a.out_row = rows[2 * (i + 1) + 1];
+      HIPCHK(ctx, rfk::launch_compat_pass(a, ctx->sp.hist_mode, ctx->stream));
+    }
+    if (num_passes % 2) std::swap(f->local.p, ctx->swap.p);
+  } else {
+    const uint64_t N = 64ull * (uint64_t)fast_waves(ctx);
+    if (P % N != 0 || (f->animated && ctx->ppt % N != 0))
+      return set_err(ctx, RF_ERR_ARG, "fast engine: particles per temporal sample must be a multiple of 64*block_waves");
+    if ((rc = ensure_perms(ctx))) return rc;
+    rfk::FastArgs a = fast_args(ctx, f);
+    a.launch = ctx->launch++;
+    a.n_iter = num_passes + 1;
+    a.first_run = 1; a.do_draw = 0;
+    a.bins = nullptr;
+    HIPCHK(ctx, rfk::launch_fast(a, P, fast_waves(ctx), ctx->sp.hist_mode, ctx->stream));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  f->have_particles = true;
+  return RF_OK;
+}
+
+static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, const uint32_t* rows_in,
+                   uint64_t* drawn) {
+  if (!ctx || !f || !h || f->ctx != ctx || h->ctx != ctx) return set_err(ctx, RF_ERR_ARG, "bad context, flame or histogram");
+  if (!f->have_particles || f->generation != ctx->generation || f->needs_update)
+    return set_err(ctx, RF_ERR_STATE, "flame needs a warmup before draw_to_bins");
+  const uint64_t P = ctx->sp.total_particles;
+  float ss[6];
+  screen_affine(f->desc, h->w, h->h, ss);
+  HIPCHK(ctx, hipMemsetAsync(ctx->counter.p, 0, 8, ctx->stream));
+  if (!ctx->ev[2]) { HIPCHK(ctx, hipEventCreate(&ctx->ev[2])); HIPCHK(ctx, hipEventCreate(&ctx->ev[3])); }
+  if (ctx->sp.engine == RF_ENGINE_COMPAT) {
+    int rc = ensure_compat(ctx);
+    if (rc) return rc;
+    std::vector<uint32_t> rows;
+    if (rows_in) rows.assign(rows_in, rows_in + 2 * (size_t)num_iter);
+    else next_rows(ctx, num_iter, rows);
+    for (auto r : rows) if (r >= ctx->shuffle_count) return set_err(ctx, RF_ERR_ARG, "shuffle row out of range");
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    rfk::CompatArgs a = compat_args(ctx, f);
+    a.random_read = 1; a.random_write = 0; a.first_run = 0; a.do_draw = 1;
+    a.bins = (float*)h->ptr;
+    a.W = h->w; a.H = h->h;
+    memcpy(a.ss, ss, sizeof(ss));
+    void* names[2] = {f->local.p, ctx->swap.p};
+    for (uint32_t i = 0; i < num_iter; ++i) {
+      a.pos_in = (const float4*)names[i % 2];
+      a.pos_out = (float4*)names[(i + 1) % 2];
+      a.in_row = rows[2 * i]; a.out_row = rows[2 * i + 1];
+      HIPCHK(ctx, rfk::launch_compat_pass(a, ctx->sp.hist_mode, ctx->stream));
+    }
+    if (num_iter % 2) std::swap(f->local.p, ctx->swap.p);
+  } else {
+    int rc = ensure_perms(ctx);
+    if (rc) return rc;
+    rfk::FastArgs a = fast_args(ctx, f);
+    a.launch = ctx->launch++;
+    a.n_iter = num_iter;
+    a.first_run = 0; a.do_draw = 1;
+    a.bins = (float*)h->ptr;
+    a.W = h->w; a.H = h->h;
+    memcpy(a.ss, ss, sizeof(ss));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    if (num_iter) HIPCHK(ctx, rfk::launch_fast(a, P, fast_waves(ctx), ctx->sp.hist_mode, ctx->stream));
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  uint64_t cnt = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->counter.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0.0f;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+  ctx->last_draw_ms = ms;
+  if (drawn) *drawn = cnt;
+  return RF_OK;
+}
+
+static rfk::DEArgs de_args(rf_hist* h, const rf_de_params* de) {
+  rfk::DEArgs a{};
+  a.hist = (const float4*)h->ptr;
+  a.W = h->w; a.H = h->h;
+  if (de) {
+    a.R = std::min(de->estimator_radius, 100);  // src/main.cpp:502
+    if (a.R < 0) a.R = 0;
+    a.rmin = de->estimator_min;
+    a.curve = de->estimator_curve;
+  }
+  return a;
+}
+
+// ================================================================== C ABI
+extern "C" {
+
+int rf_variation_count(void) { return RF_GEN_NUM_VARIATIONS; }
+const char* rf_variation_name(int id) { const rf_var_info* v = var_info(id); return v ? v->name : nullptr; }
+int rf_variation_id(const char* name) {
+  if (!name) return -1;
+  auto& idx = pindex();
+  auto it = idx.vars.find(name);
+  return it == idx.vars.end() ? -1 : it->second;
+}
+int rf_variation_num_params(int id) { const rf_var_info* v = var_info(id); return v ? v->n_params : -1; }
+const char* rf_variation_param_name(int id, int k) {
+  const rf_var_info* v = var_info(id);
+  return (v && k >= 0 && k < v->n_params) ? v->params[k] : nullptr;
+}
+int rf_variation_flags(int id) {
+  const rf_var_info* v = var_info(id);
+  return v ? (int)(v->flags | (v->uses_rand ? 0x100u : 0u)) : -1;
+}
+
+const char* rf_last_error(const rf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+double rf_last_draw_ms(const rf_ctx* ctx) { return ctx ? ctx->last_draw_ms : -1.0; }
+void* rf_stream(rf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int rf_create(const rf_sim_params* p, rf_ctx** out) {
+  if (!p || !out) return RF_ERR_ARG;
+  *out = nullptr;
+  rf_ctx* ctx = new rf_ctx();
+  ctx->sp = *p;
+  if (ctx->sp.shuffle_count == 0) ctx->sp.shuffle_count = 1024;
+  int rc = validate_sim(ctx, p->total_particles, p->temporal_samples);
+  if (rc) { delete ctx; return rc; }
+  if (hipSetDevice(p->device) != hipSuccess) { delete ctx; return RF_ERR_HIP; }
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return RF_ERR_HIP; }
+  if (ctx->counter.alloc(8) != hipSuccess) { rf_destroy(ctx); return RF_ERR_HIP; }
+  ctx->ppt = p->total_particles / p->temporal_samples;
+  ctx->shuffle_count = ctx->sp.shuffle_count;
+  ctx->row_state = p->seed ^ 0x0123456789ABCDEFull;
+  *out = ctx;
+  return RF_OK;
+}
+
+void rf_destroy(rf_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->render_hist) rf_hist_destroy(ctx->render_hist);
+  if (ctx->stream) { (void)hipStreamSynchronize(ctx->stream); (void)hipStreamDestroy(ctx->stream); }
+  for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
+  delete ctx;
+}
+
+int rf_set_sim_parameters(rf_ctx* ctx, uint64_t P, uint32_t T, uint32_t R) {
+  if (!ctx) return RF_ERR_ARG;
+  int rc = validate_sim(ctx, P, T);
+  if (rc) return rc;
+  ctx->sp.total_particles = P;
+  ctx->sp.temporal_samples = T;
+  ctx->sp.shuffle_count = R ? R : 1024;
+  ctx->shuffle_count = ctx->sp.shuffle_count;
+  ctx->ppt = P / T;
+  ctx->swap.release(); ctx->rs.release(); ctx->shuf.release(); ctx->samples.release();
+  ctx->compat_ready = false;
+  ctx->generation++;  // invalidates every flame's particle state (src/flame.cpp:153-157)
+  return RF_OK;
+}
+
+int rf_set_engine(rf_ctx* ctx, int32_t engine, int32_t hist_mode) {
+  if (!ctx) return RF_ERR_ARG;
+  if (engine != RF_ENGINE_COMPAT && engine != RF_ENGINE_FAST) return set_err(ctx, RF_ERR_ARG, "unknown engine");
+  if (hist_mode < 0 || hist_mode > 3) return set_err(ctx, RF_ERR_ARG, "unknown hist mode");
+  if (engine != ctx->sp.engine) ctx->generation++;  // particle layouts differ between engines
+  ctx->sp.engine = engine;
+  ctx->sp.hist_mode = hist_mode;
+  return RF_OK;
+}
+
+int rf_flame_parse_desc(const char* xml, size_t len, rf_flame_desc* out, char* err, size_t err_len) {
+  if (!xml || !out) return RF_ERR_ARG;
+  std::string e;
+  int rc = parse_flam3(xml, len, out, e);
+  if (rc && err && err_len) { snprintf(err, err_len, "%s", e.c_str()); }
+  return rc;
+}
+
+int rf_flame_create(rf_ctx* ctx, const rf_flame_desc* desc, rf_flame** out) {
+  if (!ctx || !desc || !out) return set_err(ctx, RF_ERR_ARG, "null argument");
+  *out = nullptr;
+  int rc = check_desc(ctx, desc);
+  if (rc) return rc;
+  rf_flame* f = new rf_flame();
+  f->ctx = ctx;
+  f->desc = *desc;
+  f->generation = ctx->generation;
+  *out = f;
+  return RF_OK;
+}
+
+int rf_flame_parse(rf_ctx* ctx, const char* xml, size_t len, rf_flame** out) {
+  if (!ctx || !xml || !out) return set_err(ctx, RF_ERR_ARG, "null argument");
+  rf_flame_desc* d = new rf_flame_desc();
+  std::string e;
+  int rc = parse_flam3(xml, len, d, e);
+  if (rc) { delete d; return set_err(ctx, rc, e); }
+  rc = rf_flame_create(ctx, d, out);
+  delete d;
+  return rc;
+}
+
+int rf_flame_load(rf_ctx* ctx, const char* path, rf_flame** out) {
+  if (!ctx || !path || !out) return set_err(ctx, RF_ERR_ARG, "null argument");
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return set_err(ctx, RF_ERR_IO, std::string("cannot open ") + path);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  const std::string s = ss.str();
+  return rf_flame_parse(ctx, s.data(), s.size(), out);
+}
+
+int rf_flame_update(rf_flame* f, const rf_flame_desc* desc) {
+  if (!f || !desc) return RF_ERR_ARG;
+  int rc = check_desc(f->ctx, desc);
+  if (rc) return rc;
+  f->desc = *desc;
+  f->needs_update = true;
+  return RF_OK;
+}
+
+int rf_flame_get_desc(const rf_flame* f, rf_flame_desc* out) {
+  if (!f || !out) return RF_ERR_ARG;
+  *out = f->desc;
+  return RF_OK;
+}
+
+void rf_flame_destroy(rf_flame* f) { delete f; }
+
+int rf_flame_param_block(const rf_flame_desc* d, float* out, int cap) {
+  if (!d) return RF_ERR_ARG;
+  std::vector<float> v = reference_param_block(*d);
+  if (out) for (int i = 0; i < cap && i < (int)v.size(); ++i) out[i] = v[i];
+  return (int)v.size();
+}
+
+int rf_flame_validate(const rf_flame_desc* d, char* err, size_t err_len) {
+  if (!d) return RF_ERR_ARG;
+  Block* b = new Block();
+  std::string e;
+  bool ok = build_block(*d, *b, e);
+  delete b;
+  if (!ok && err && err_len) snprintf(err, err_len, "%s", e.c_str());
+  return ok ? RF_OK : RF_ERR_ARG;
+}
+
+int rf_screen_affine(const rf_flame_desc* d, uint32_t w, uint32_t h, float out[6]) {
+  if (!d || !out) return RF_ERR_ARG;
+  screen_affine(*d, w, h, out);
+  return RF_OK;
+}
+
+int rf_hist_create(rf_ctx* ctx, uint32_t w, uint32_t h, rf_hist** out) {
+  if (!ctx || !out || w == 0 || h == 0) return set_err(ctx, RF_ERR_ARG, "bad histogram size");
+  if ((uint64_t)w * h * 4 > 0xffffffffull) return set_err(ctx, RF_ERR_LIMIT, "histogram larger than 2^30 bins");
+  rf_hist* hs = new rf_hist();
+  hs->ctx = ctx; hs->w = w; hs->h = h; hs->owned = true;
+  hipError_t e = hs->own.alloc((size_t)w * h * 16);
+  if (e != hipSuccess) { delete hs; return set_err(ctx, RF_ERR_HIP, std::string("hist alloc: ") + hipGetErrorString(e)); }
+  hs->ptr = hs->own.p;
+  e = hipMemsetAsync(hs->ptr, 0, (size_t)w * h * 16, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) { delete hs; return set_err(ctx, RF_ERR_HIP, hipGetErrorString(e)); }
+  *out = hs;
+  return RF_OK;
+}
+
+int rf_hist_wrap(rf_ctx* ctx, void* dev, uint32_t w, uint32_t h, rf_hist** out) {
+  if (!ctx || !dev || !out || w == 0 || h == 0) return set_err(ctx, RF_ERR_ARG, "bad histogram");
+  if ((uint64_t)w * h * 4 > 0xffffffffull) return set_err(ctx, RF_ERR_LIMIT, "histogram larger than 2^30 bins");
+  rf_hist* hs = new rf_hist();
+  hs->ctx = ctx; hs->w = w; hs->h = h; hs->ptr = dev; hs->owned = false;
+  *out = hs;
+  return RF_OK;
+}
+
+void rf_hist_destroy(rf_hist* h) { delete h; }
+
+int rf_hist_zero(rf_hist* h) {
+  if (!h) return RF_ERR_ARG;
+  HIPCHK(h->ctx, hipMemsetAsync(h->ptr, 0, (size_t)h->w * h->h * 16, h->ctx->stream));
+  HIPCHK(h->ctx, hipStreamSynchronize(h->ctx->stream));
+  return RF_OK;
+}
+
+int rf_hist_download(rf_hist* h, float* host) {
+  if (!h || !host) return RF_ERR_ARG;
+  HIPCHK(h->ctx, hipMemcpyAsync(host, h->ptr, (size_t)h->w * h->h * 16, hipMemcpyDeviceToHost, h->ctx->stream));
+  HIPCHK(h->ctx, hipStreamSynchronize(h->ctx->stream));
+  return RF_OK;
+}
+
+int rf_hist_upload(rf_hist* h, const float* host) {
+  if (!h || !host) return RF_ERR_ARG;
+  HIPCHK(h->ctx, hipMemcpyAsync(h->ptr, host, (size_t)h->w * h->h * 16, hipMemcpyHostToDevice, h->ctx->stream));
+  HIPCHK(h->ctx, hipStreamSynchronize(h->ctx->stream));
+  return RF_OK;
+}
+
+void* rf_hist_device_ptr(rf_hist* h) { return h ? h->ptr : nullptr; }
+
+int rf_warmup(rf_ctx* ctx, rf_flame* f, uint32_t n, float tss) { return do_warmup(ctx, f, n, tss, nullptr); }
+int rf_warmup_rows(rf_ctx* ctx, rf_flame* f, uint32_t n, float tss, const uint32_t* rows) {
+  if (!rows) return set_err(ctx, RF_ERR_ARG, "rows is null");
+  return do_warmup(ctx, f, n, tss, rows);
+}
+int rf_draw_to_bins(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t n, uint64_t* drawn) {
+  return do_draw(ctx, f, h, n, nullptr, drawn);
+}
+int rf_draw_to_bins_rows(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t n, const uint32_t* rows, uint64_t* drawn) {
+  if (!rows) return set_err(ctx, RF_ERR_ARG, "rows is null");
+  return do_draw(ctx, f, h, n, rows, drawn);
+}
+
+int rf_density_estimate(rf_ctx* ctx, rf_hist* h, const rf_de_params* de, void* out) {
+  if (!ctx || !h || !de) return set_err(ctx, RF_ERR_ARG, "null argument");
+  if (!out) { HIPCHK(ctx, h->image.alloc((size_t)h->w * h->h * 16)); out = h->image.p; }
+  rfk::DEArgs a = de_args(h, de);
+  a.de_out = (float4*)out;
+  a.do_tonemap = 0;
+  HIPCHK(ctx, rfk::launch_de_tonemap(a, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return RF_OK;
+}
+
+int rf_tonemap(rf_ctx* ctx, const void* in, void* out, uint32_t w, uint32_t h, const rf_tonemap_params* tm) {
+  if (!ctx || !in || !out || !tm) return set_err(ctx, RF_ERR_ARG, "null argument");
+  HIPCHK(ctx, rfk::launch_tonemap((const float4*)in, (float4*)out, w * h, tm->gamma, tm->brightness, tm->vibrancy,
+                                  tm->scale_constant, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return RF_OK;
+}
+
+int rf_density_tonemap(rf_ctx* ctx, rf_hist* h, const rf_de_params* de, const rf_tonemap_params* tm, void* out) {
+  if (!ctx || !h || !tm) return set_err(ctx, RF_ERR_ARG, "null argument");
+  if (!out) { HIPCHK(ctx, h->image.alloc((size_t)h->w * h->h * 16)); out = h->image.p; }
+  rfk::DEArgs a = de_args(h, de);
+  a.img_out = (float4*)out;
+  a.do_tonemap = 1;
+  a.gamma = tm->gamma; a.brightness = tm->brightness; a.vibrancy = tm->vibrancy; a.k = tm->scale_constant;
+  HIPCHK(ctx, rfk::launch_de_tonemap(a, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return RF_OK;
+}
+
+int rf_render(rf_ctx* ctx, rf_flame* f, const rf_render_params* rp, float* hist_out, float* image_out, rf_stats* st) {
+  if (!ctx || !f || !rp) return set_err(ctx, RF_ERR_ARG, "null argument");
+  using clk = std::chrono::steady_clock;
+  auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  if (!ctx->render_hist || ctx->render_hist->w != rp->width || ctx->render_hist->h != rp->height) {
+    if (ctx->render_hist) { rf_hist_destroy(ctx->render_hist); ctx->render_hist = nullptr; }
+    int rc = rf_hist_create(ctx, rp->width, rp->height, &ctx->render_hist);
+    if (rc) return rc;
+  }
+  rf_hist* h = ctx->render_hist;
+  int rc = rf_hist_zero(h);
+  if (rc) return rc;
+  rf_stats s{};
+  auto t0 = clk::now();
+  if ((rc = rf_warmup(ctx, f, rp->warmup_passes, rp->tss_width))) return rc;
+  auto t1 = clk::now();
+  for (uint32_t k = 0; k < rp->draw_calls; ++k) {
+    uint64_t d = 0;
+    if ((rc = rf_draw_to_bins(ctx, f, h, rp->draw_passes, &d))) return rc;
+    s.drawn += d;
+  }
+  auto t2 = clk::now();
+  rf_de_params none{0, 0, 0.0f};
+  HIPCHK(ctx, ctx->render_img.alloc((size_t)rp->width * rp->height * 16));
+  if ((rc = rf_density_tonemap(ctx, h, rp->density_estimation ? &rp->de : &none, &rp->tm, ctx->render_img.p))) return rc;
+  auto t3 = clk::now();
+  if (hist_out) if ((rc = rf_hist_download(h, hist_out))) return rc;
+  if (image_out) {
+    HIPCHK(ctx, hipMemcpy(image_out, ctx->render_img.p, (size_t)rp->width * rp->height * 16, hipMemcpyDeviceToHost));
+  }
+  s.iterations = ctx->sp.total_particles * ((uint64_t)rp->warmup_passes + 1 + (uint64_t)rp->draw_passes * rp->draw_calls);
+  s.ms_warmup = ms(t0, t1);
+  s.ms_draw = ms(t1, t2);
+  s.ms_density = ms(t2, t3);
+  s.ms_tonemap = 0.0;
+  if (st) *st = s;
+  return RF_OK;
+}
+
+int rf_particles_download(rf_ctx* ctx, rf_flame* f, float* host) {
+  if (!ctx || !f || !host || !f->local.p) return set_err(ctx, RF_ERR_ARG, "no particles");
+  HIPCHK(ctx, hipMemcpy(host, f->local.p, ctx->sp.total_particles * 16, hipMemcpyDeviceToHost));
+  return RF_OK;
+}
+int rf_rng_download(rf_ctx* ctx, uint32_t* host) {
+  if (!ctx || !host) return RF_ERR_ARG;
+  int rc = ensure_compat(ctx);
+  if (rc) return rc;
+  HIPCHK(ctx, hipMemcpy(host, ctx->rs.p, ctx->sp.total_particles * 16, hipMemcpyDeviceToHost));
+  return RF_OK;
+}
+int rf_shuffle_download(rf_ctx* ctx, uint32_t* host) {
+  if (!ctx || !host) return RF_ERR_ARG;
+  int rc = ensure_compat(ctx);
+  if (rc) return rc;
+  HIPCHK(ctx, hipMemcpy(host, ctx->shuf.p, ctx->shuf.bytes, hipMemcpyDeviceToHost));
+  return RF_OK;
+}
+int rf_sample_points_download(rf_ctx* ctx, float* host) {
+  if (!ctx || !host) return RF_ERR_ARG;
+  int rc = ensure_compat(ctx);
+  if (rc) return rc;
+  HIPCHK(ctx, hipMemcpy(host, ctx->samples.p, ctx->samples.bytes, hipMemcpyDeviceToHost));
+  return RF_OK;
+}
+
+int rf_fast_perms_download(rf_ctx* ctx, uint16_t* host) {
+  if (!ctx || !host) return RF_ERR_ARG;
+  int rc = ensure_perms(ctx);
+  if (rc) return rc;
+  HIPCHK(ctx, hipMemcpy(host, ctx->perms.p, ctx->perms.bytes, hipMemcpyDeviceToHost));
+  return RF_OK;
+}
+
+int rf_eval_xform(rf_ctx* ctx, const rf_xform* xf, const float* pts, const float* rands, float* out, uint32_t n) {
+  if (!ctx || !xf || !pts || !rands || !out) return set_err(ctx, RF_ERR_ARG, "null argument");
+  rf_flame_desc* d = new rf_flame_desc();
+  memset(d, 0, sizeof(*d));
+  d->n_xforms = 1;
+  d->xforms[0] = *xf;
+  d->xforms[0].weight = 1.0f;
+  Block b;
+  std::string err;
+  bool ok = build_block(*d, b, err);
+  delete d;
+  if (!ok) return set_err(ctx, RF_ERR_ARG, err);
+  DevBuf db, dp, dr, dout;
+  HIPCHK(ctx, db.alloc(sizeof(Block)));
+  HIPCHK(ctx, dp.alloc((size_t)n * 12 + 16));
+  HIPCHK(ctx, dr.alloc((size_t)n * 64 + 16));
+  HIPCHK(ctx, dout.alloc((size_t)n * 16 + 16));
+  HIPCHK(ctx, hipMemcpy(db.p, &b, sizeof(Block), hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(dp.p, pts, (size_t)n * 12, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(dr.p, rands, (size_t)n * 64, hipMemcpyHostToDevice));
+  HIPCHK(ctx, rfk::launch_eval_xform((const Block*)db.p, (const float*)dp.p, (const float*)dr.p, (float*)dout.p, n, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(out, dout.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+  return RF_OK;
+}
+
+size_t rf_struct_size(int which) {
+  switch (which) {
+    case 0: return sizeof(rf_var_term);
+    case 1: return sizeof(rf_xform);
+    case 2: return sizeof(rf_flame_desc);
+    case 3: return sizeof(rf_sim_params);
+    case 4: return sizeof(rf_de_params);
+    case 5: return sizeof(rf_tonemap_params);
+    case 6: return sizeof(rf_render_params);
+    case 7: return sizeof(rf_stats);
+    default: return 0;
+  }
+}
+
+int rf_atomic_probe(rf_ctx* ctx, int32_t hist_mode, uint64_t footprint_bins, uint64_t samples, double* ms_out) {
+  if (!ctx || !ms_out || footprint_bins == 0) return set_err(ctx, RF_ERR_ARG, "bad probe arguments");
+  if (footprint_bins > (1ull << 28)) return set_err(ctx, RF_ERR_LIMIT, "probe footprint above 2^28 bins");
+  if (samples < 64 * 256 || samples % (64 * 256) != 0) return set_err(ctx, RF_ERR_ARG, "samples must be a multiple of 16384");
+  HIPCHK(ctx, ctx->probe.alloc(footprint_bins * 16 + 256 * 16));
+  float4* pal = (float4*)((char*)ctx->probe.p + footprint_bins * 16);
+  HIPCHK(ctx, hipMemsetAsync(ctx->probe.p, 0, footprint_bins * 16 + 256 * 16, ctx->stream));
+  if (!ctx->ev[0]) { HIPCHK(ctx, hipEventCreate(&ctx->ev[0])); HIPCHK(ctx, hipEventCreate(&ctx->ev[1])); }
+  HIPCHK(ctx, rfk::launch_atomic_probe((float*)ctx->probe.p, footprint_bins, samples, hist_mode, pal, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  HIPCHK(ctx, rfk::launch_atomic_probe((float*)ctx->probe.p, footprint_bins, samples, hist_mode, pal, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+  float ms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  *ms_out = ms;
+  return RF_OK;
+}
+
+}  // extern "C"

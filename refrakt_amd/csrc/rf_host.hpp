@@ -1,0 +1,95 @@
+// Host-side objects behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/refrakt_amd.h"
+#include "rf_device.hpp"
+
+struct rf_var_info {
+  const char* name;
+  int n_params;
+  const char* params[RF_MAX_VAR_PARAMS];
+  uint32_t flags;     // 1 no_weight_mul, 2 pre_xform
+  uint32_t precalc;
+  int uses_rand;
+  int ref_line;       // line of the entry in the reference's variations.yaml
+};
+
+namespace rfh {
+
+int parse_flam3(const char* xml, size_t len, rf_flame_desc* out, std::string& err);
+bool param_lookup(const char* name, int* var, int* k);
+const rf_var_info* var_info(int id);
+
+// Device buffer owned by the library.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t alloc(size_t n);
+  void release();
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// Builds the device Block of a flame (weights normalised over the non-final
+// xforms, terms in name order, pre_xform terms first).  Returns false and
+// sets err on a limit violation.
+bool build_block(const rf_flame_desc& d, rfd::Block& b, std::string& err);
+// The reference's flat parameter block (src/flame.cpp:33-103).
+std::vector<float> reference_param_block(const rf_flame_desc& d);
+// Screen-space affine (src/flame.cpp:289-296, src/flame.hpp:97-128).
+void screen_affine(const rf_flame_desc& d, uint32_t w, uint32_t h, float out[6]);
+
+}  // namespace rfh
+
+struct rf_ctx {
+  rf_sim_params sp{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  uint64_t ppt = 0;            // particles per temporal sample
+  uint32_t shuffle_count = 0;
+  uint64_t generation = 1;     // bumped by rf_set_sim_parameters
+  uint32_t launch = 0;         // fast-engine launch counter
+  uint64_t row_state = 0;      // compat-engine shuffle-row schedule stream
+  // compat engine buffers
+  rfh::DevBuf swap, rs, shuf, samples;
+  bool compat_ready = false;
+  // fast engine
+  rfh::DevBuf perms;
+  int perm_waves = 0;
+  rfh::DevBuf counter;         // u64 drawn counter
+  rfh::DevBuf probe;           // atomic probe scratch
+  rf_hist* render_hist = nullptr;
+  rfh::DevBuf render_img;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_draw_ms = 0.0;   // device time of the last draw_to_bins' chaos-game launches
+};
+
+struct rf_flame {
+  rf_ctx* ctx = nullptr;
+  rf_flame_desc desc{};
+  uint64_t generation = 0;     // ctx generation the particle state belongs to
+  bool needs_update = true;
+  bool animated = false;
+  uint32_t n_blocks = 1;       // blocks read by the kernels (1 or T)
+  rfh::DevBuf blocks;          // n_blocks (+1 pristine when animated) Blocks
+  rfh::DevBuf rot;             // rotation frequencies (animated)
+  rfh::DevBuf palette;         // 256 float4
+  rfh::DevBuf local;           // particles
+  bool have_particles = false;
+};
+
+struct rf_hist {
+  rf_ctx* ctx = nullptr;
+  uint32_t w = 0, h = 0;
+  void* ptr = nullptr;
+  bool owned = false;
+  rfh::DevBuf own;
+  rfh::DevBuf image;           // density-estimated / tonemapped output
+};

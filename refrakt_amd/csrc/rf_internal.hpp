@@ -1,0 +1,71 @@
+// Internal interface between the host library (rf_host.cpp) and the kernel
+// launchers (rf_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rf_device.hpp"
+
+namespace rfk {
+
+struct CompatArgs {
+  const float4* pos_in;
+  float4* pos_out;
+  const uint32_t* shuf;          // shuffle rows, row-major, ppt entries per row
+  uint4* rs;                     // jsf32 state per slot
+  const rfd::Block* blocks;      // per temporal sample
+  uint32_t block_stride;         // 0: one block for all samples, 1: one per sample
+  const float4* palette;
+  float* bins;                   // W*H RGBA
+  unsigned long long* counter;   // binned samples
+  uint32_t ppt;                  // particles per temporal sample
+  uint32_t T;                    // temporal samples
+  uint32_t in_row, out_row;
+  uint32_t random_read, random_write, first_run, do_draw;
+  uint32_t W, H;
+  float ss[6];                   // screen affine
+};
+
+struct FastArgs {
+  float4* pos;                   // P particles (x, y, c, 0), read and written
+  const rfd::Block* blocks;
+  uint32_t block_stride;
+  const float4* palette;
+  float* bins;
+  unsigned long long* counter;
+  const uint16_t* perms;         // NPERM rows x (64*waves) slot permutation
+  uint64_t seed;
+  uint32_t launch;               // launch counter (RNG streams)
+  uint32_t ppt;
+  uint32_t n_iter;
+  uint32_t first_run, do_draw;
+  uint32_t W, H;
+  float ss[6];
+};
+
+struct DEArgs {
+  const float4* hist;            // W*H bins (rows bottom-up)
+  float4* de_out;                // may be null
+  float4* img_out;               // may be null (tonemapped)
+  uint32_t W, H;
+  int32_t R, rmin;
+  float curve;
+  int32_t do_tonemap;
+  float gamma, brightness, vibrancy, k;
+};
+
+constexpr int NPERM = 256;
+
+hipError_t launch_init_rs(uint4* rs, uint64_t n, hipStream_t s);
+hipError_t launch_compat_pass(const CompatArgs& a, int hist_mode, hipStream_t s);
+hipError_t launch_fast(const FastArgs& a, uint64_t n_particles, int waves, int hist_mode, hipStream_t s);
+hipError_t launch_animate(rfd::Block* blocks, uint32_t T, float tss_width, const float* rot, hipStream_t s);
+hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s);
+hipError_t launch_tonemap(const float4* in, float4* out, uint32_t n, float gamma, float brightness,
+                          float vibrancy, float k, hipStream_t s);
+hipError_t launch_eval_xform(const rfd::Block* blk, const float* pts, const float* rands, float* out,
+                             uint32_t n, hipStream_t s);
+hipError_t launch_atomic_probe(float* bins, uint64_t nbins, uint64_t samples, int hist_mode,
+                               const float4* palette, hipStream_t s);
+
+}  // namespace rfk

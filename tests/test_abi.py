@@ -1,0 +1,68 @@
+"""The C-ABI library loads on the CPU and exports every symbol of include/refrakt_amd.h."""
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "refrakt_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(rf_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(rf):
+    L = rf.lib()
+    declared = _declared()
+    assert len(declared) > 40
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    # the ctypes binding covers the whole header
+    from refrakt_amd import api
+    assert sorted(api.exported_symbols()) == declared
+
+
+def test_struct_layouts_match(rf):
+    from refrakt_amd import api
+    L = rf.lib()
+    for i, st in enumerate(api._STRUCTS):
+        assert L.rf_struct_size(i) == api.C.sizeof(st)
+
+
+def test_variation_registry_matches_oracle_table(rf, orc):
+    names = rf.variation_names()
+    assert len(names) == 76
+    L = orc.lib()
+    onames = [L.orc_var_name(i).decode() for i in range(L.orc_var_count())]
+    assert sorted(names) == sorted(onames)
+    for n in names:
+        assert rf.variation_params(n) == orc.var_params(n), n
+    assert rf.variation_id("no_such_variation") == -1
+    # flags: radial_blur skips the weight multiply, pre_blur is applied before the precalcs
+    L2 = rf.lib()
+    assert L2.rf_variation_flags(rf.variation_id("radial_blur")) & 1
+    assert L2.rf_variation_flags(rf.variation_id("pre_blur")) & 2
+    assert L2.rf_variation_flags(rf.variation_id("julia")) & 0x100
+    assert not L2.rf_variation_flags(rf.variation_id("linear")) & 0x100
+
+
+def test_generated_tables_are_current():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "refrakt_amd", "gen_variations.py"), "--check"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_context_creation_fails_loudly_without_gpu(rf):
+    import torch
+    if torch.cuda.is_available():
+        return
+    try:
+        rf.Context(1 << 10, 1)
+    except rf.RefraktError as e:
+        assert "rf_create" in str(e)
+    else:
+        raise AssertionError("rf_create succeeded without a GPU")

@@ -1,0 +1,298 @@
+"""GPU parity: the HIP path against the CPU oracle, through the C ABI.
+
+Tolerances (stated per test):
+  * bin indices, hit counts, drawn counts, particle positions and RNG states of
+    affine/linear-only flames: bit-exact;
+  * histogram colour sums: relative 1e-5 (float atomics add in any order);
+  * single-xform evaluation vs the FP64 oracle: |d| <= 2e-4 * (1 + |ref|) on
+    at least 99.5% of points (98% for variations with floor/round/trunc
+    discontinuities, where float and double can land on either side);
+  * density estimation vs the FP32-restated oracle: 2e-5 of the image maximum;
+  * tonemap vs the FP64 oracle: 2e-5 absolute on [0, 1] outputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _linear_flame(rf, seed, n=3, final=True, post=True, W=256, H=256):
+    rng = np.random.default_rng(seed)
+    xs = []
+    for _ in range(n):
+        th = rng.uniform(0, 2 * math.pi)
+        s = rng.uniform(0.35, 0.8)
+        aff = [s * math.cos(th), s * math.sin(th), -s * math.sin(th), s * math.cos(th)] + list(rng.uniform(-0.8, 0.8, 2))
+        xs.append(rf.Xform(affine=aff, post=list(rng.uniform(-1, 1, 6)) if post and rng.uniform() < 0.5 else None,
+                           variations={"linear": float(rng.uniform(0.6, 1.2))}, weight=float(rng.uniform(0.2, 1)),
+                           color=float(rng.uniform()), color_speed=0.5, opacity=1.0))
+    fin = None
+    if final:
+        fin = rf.Xform(affine=[0.9, 0.1, -0.1, 0.9, 0.05, -0.02], variations={"linear": 1.0}, color=0.3,
+                       color_speed=0.2, opacity=1.0)
+    pal = np.ones((256, 4), np.float32)
+    pal[:, 0] = np.linspace(0, 1, 256)
+    pal[:, 1] = np.linspace(1, 0, 256)
+    pal[:, 2] = 0.5
+    return rf.Flame(xforms=xs, final_xform=fin, palette=pal, size=[W, H], center=[0.0, 0.0], scale=W / 5.0)
+
+
+def _ctx(rf, P, T, engine, hist_mode=1, R=16, seed=7, waves=4):
+    return rf.Context(P, T, R, seed=seed, engine=engine, hist_mode=hist_mode, block_waves=waves)
+
+
+# ------------------------------------------------------------ compat engine
+def test_compat_bit_exact_vs_oracle(rf, orc, gpu_available):
+    P, T, R, W, H = 1 << 14, 4, 16, 256, 256
+    ppt = P // T
+    fl = _linear_flame(rf, 11, W=W, H=H)
+    ctx = _ctx(rf, P, T, rf.RF_ENGINE_COMPAT, R=R)
+    rows_rng = np.random.default_rng(5)
+    warm = 3
+    wrows = rows_rng.integers(0, R, size=2 * (warm + 1)).astype(np.uint32)
+    drows = rows_rng.integers(0, R, size=2 * 5).astype(np.uint32)
+    fh = ctx.flame(fl)
+    hist = ctx.histogram(W, H)
+    fh.warmup(warm, rows=wrows)
+    drawn = fh.draw_to_bins(hist, 5, rows=drows)
+    g_hist = hist.download()
+    g_pos = fh.particles()
+    g_rs = ctx.rng_states()
+
+    shuf = ctx.shuffle_rows()
+    assert shuf.shape == (R, ppt)
+    assert np.array_equal(np.sort(shuf, axis=1), np.tile(np.arange(ppt, dtype=np.uint32), (R, 1)))
+    samples = orc.hammersley(ppt)
+    assert np.array_equal(ctx.sample_points()[:, :2].view(np.uint32), samples[:, :2].view(np.uint32))
+
+    fc = orc.flame_to_c(fl)
+    rs = orc.jsf_seed_states(P)
+    bufs = [np.zeros((P, 4), np.float32), np.zeros((P, 4), np.float32)]
+    orc.compat_pass(fc, T, ppt, samples, bufs[0], shuf, rs, wrows[0], wrows[1], 1, 1, 1, 0)
+    for i in range(warm):
+        orc.compat_pass(fc, T, ppt, bufs[i % 2], bufs[(i + 1) % 2], shuf, rs, wrows[2 + 2 * i], wrows[3 + 2 * i],
+                        1, 1, 0, 0)
+    if warm % 2:
+        bufs = [bufs[1], bufs[0]]
+    ss = orc.screen_affine(fl, W, H)
+    assert np.array_equal(ss, rf.screen_affine(fl, W, H))
+    bins = np.zeros((H, W, 4), np.float64)
+    hits = np.zeros((H, W), np.uint32)
+    o_drawn = 0
+    for i in range(5):
+        o_drawn += orc.compat_pass(fc, T, ppt, bufs[i % 2], bufs[(i + 1) % 2], shuf, rs, drows[2 * i],
+                                   drows[2 * i + 1], 1, 0, 0, 1, W, H, ss, bins, hits)
+    final = bufs[1] if 5 % 2 else bufs[0]
+
+    assert drawn == o_drawn > 0
+    assert np.array_equal(g_pos.view(np.uint32), final.view(np.uint32)), "particle positions differ"
+    assert np.array_equal(g_rs, rs), "jsf32 states differ"
+    assert np.array_equal(g_hist[..., 3].astype(np.uint32), hits), "hit counts (bin indices) differ"
+    np.testing.assert_allclose(g_hist[..., :3], bins[..., :3], rtol=1e-5, atol=1e-5)
+
+
+# -------------------------------------------------------------- fast engine
+@pytest.mark.parametrize("hist_mode", [1, 2, 3])
+def test_fast_bit_exact_vs_oracle_replica(rf, orc, gpu_available, hist_mode):
+    P, T, W, H, seed = 1 << 13, 1, 192, 160, 99
+    fl = _linear_flame(rf, 21, W=W, H=H)
+    ctx = _ctx(rf, P, T, rf.RF_ENGINE_FAST, hist_mode=hist_mode, seed=seed)
+    fh = ctx.flame(fl)
+    hist = ctx.histogram(W, H)
+    fh.warmup(2)
+    drawn = fh.draw_to_bins(hist, 6)
+    g_hist = hist.download()
+    g_pos = fh.particles()
+    perms = ctx.fast_perms(4)
+    assert np.array_equal(np.sort(perms, axis=1), np.tile(np.arange(256, dtype=np.uint16), (256, 1)))
+
+    fc = orc.flame_to_c(fl)
+    ss = orc.screen_affine(fl, W, H)
+    pos = np.zeros((P, 4), np.float32)
+    orc.fast_run(fc, P // T, P, 4, perms, seed, 0, 3, True, False, W, H, ss, pos)
+    bins = np.zeros((H, W, 4), np.float64)
+    hits = np.zeros((H, W), np.uint32)
+    o_drawn = orc.fast_run(fc, P // T, P, 4, perms, seed, 1, 6, False, True, W, H, ss, pos, bins, hits)
+    assert drawn == o_drawn > 0
+    assert np.array_equal(g_pos.view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(g_hist[..., 3].astype(np.uint32), hits)
+    np.testing.assert_allclose(g_hist[..., :3], bins[..., :3], rtol=1e-5, atol=1e-5)
+
+
+# --------------------------------------------------------------- variations
+DISCONTINUOUS = {"rings", "fan", "fan2", "rings2", "julian", "juliascope", "pie", "ngon", "rectangles", "boarders",
+                 "cell", "cpow", "secant2", "twintrian", "bipolar", "oscope", "julia", "loonie", "bent", "bent2",
+                 "tangent", "popcorn", "rays", "arch", "super_shape", "flower", "conic", "disc2", "curl", "foci",
+                 "coth", "mobius", "perspective", "cross", "edisc"}
+
+
+def _var_params(name, rng):
+    pr = {
+        "blob_high": 1.2, "blob_low": 0.4, "blob_waves": 3.0, "pdj_a": 1.1, "pdj_b": -0.7, "pdj_c": 0.9,
+        "pdj_d": 1.6, "fan2_x": 0.6, "fan2_y": 0.3, "rings2_val": 0.7, "perspective_angle": 0.4,
+        "perspective_dist": 2.5, "julian_power": 5.0, "julian_dist": 1.3, "juliascope_power": 3.0,
+        "juliascope_dist": 0.8, "radial_blur_angle": 0.6, "pie_slices": 6.0, "pie_rotation": 0.5,
+        "pie_thickness": 0.4, "ngon_power": 2.0, "ngon_sides": 5.0, "ngon_corners": 1.0, "ngon_circle": 0.5,
+        "curl_c1": 0.4, "curl_c2": 0.2, "rectangles_x": 0.6, "rectangles_y": 0.4, "disc2_rot": 0.7,
+        "disc2_twist": 1.1, "super_shape_n1": 2.0, "super_shape_n2": 1.5, "super_shape_n3": 1.2,
+        "super_shape_m": 4.0, "super_shape_holes": 0.2, "super_shape_rnd": 0.3, "flower_holes": 0.3,
+        "flower_petals": 5.0, "conic_holes": 0.2, "conic_eccentricity": 0.7, "parabola_height": 0.8,
+        "parabola_width": 1.2, "bent2_x": 1.4, "bent2_y": 0.6, "bipolar_shift": 0.3, "cell_size": 0.7,
+        "cpow_power": 3.0, "cpow_r": 1.2, "cpow_i": 0.3, "curve_xlength": 1.1, "curve_ylength": 0.8,
+        "curve_xamp": 0.5, "curve_yamp": -0.4, "oscope_separation": 0.5, "oscope_frequency": 1.5,
+        "oscope_amplitude": 0.8, "oscope_damping": 0.3, "waves2_scalex": 0.4, "waves2_scaley": 0.3,
+        "waves2_freqx": 2.0, "waves2_freqy": 1.5, "auger_freq": 3.0, "auger_scale": 0.5, "auger_sym": 0.4,
+        "auger_weight": 0.6, "flux_spread": 0.5, "mobius_re_a": 0.9, "mobius_re_b": 0.2, "mobius_re_c": 0.3,
+        "mobius_re_d": 1.1, "mobius_im_a": 0.1, "mobius_im_b": -0.3, "mobius_im_c": 0.25, "mobius_im_d": 0.05,
+    }
+    return pr
+
+
+def test_every_variation_vs_fp64_oracle(rf, orc, gpu_available):
+    ctx = _ctx(rf, 1 << 10, 1, rf.RF_ENGINE_FAST)
+    rng = np.random.default_rng(3)
+    n = 4096
+    failures = []
+    names = rf.variation_names()
+    assert sorted(names) == sorted(orc.lib().orc_var_name(i).decode() for i in range(orc.lib().orc_var_count()))
+    for name in names:
+        prm = {p: v for p, v in _var_params(name, rng).items() if p in rf.variation_params(name)}
+        xf = rf.Xform(affine=[0.8, 0.3, -0.2, 0.9, 0.1, -0.05], post=[1.0, 0.1, 0.0, 0.9, 0.02, 0.0],
+                      variations={name: 0.7}, var_param=prm, color=0.4, color_speed=0.3, opacity=0.8)
+        if name == "pre_blur":  # a pre_xform needs a regular term to produce a point
+            xf.variations["linear"] = 1.0
+        pts = np.concatenate([rng.uniform(-1.5, 1.5, (n, 2)), rng.uniform(0, 1, (n, 1))], axis=1).astype(np.float32)
+        rands = rng.uniform(0, 1, (n, 16)).astype(np.float32)
+        g = ctx.eval_xform(xf, pts, rands)
+        o = orc.apply_f64(xf, pts.astype(np.float64), rands)
+        fin = np.isfinite(o[:, :2]).all(axis=1) & (np.abs(o[:, :2]) < 1e4).all(axis=1)
+        err = np.abs(g[:, :2].astype(np.float64) - o[:, :2]) / (1.0 + np.abs(o[:, :2]))
+        ok = (err <= 2e-4).all(axis=1) & fin
+        frac = ok.sum() / max(fin.sum(), 1)
+        need = 0.98 if name in DISCONTINUOUS else 0.995
+        if frac < need:
+            failures.append((name, float(frac)))
+        np.testing.assert_allclose(g[:, 2], o[:, 2], rtol=1e-6, atol=1e-6)
+        assert np.all(g[:, 3] == np.float32(0.8))
+    assert not failures, failures
+
+
+# ------------------------------------------------------------- DE, tonemap
+def _sparse_hist(rng, W, H, density=0.3):
+    h = np.zeros((H, W, 4), np.float32)
+    m = rng.uniform(size=(H, W)) < density
+    a = np.exp(rng.uniform(-1, 7, size=(H, W))).astype(np.float32)
+    h[..., 3] = np.where(m, a, 0)
+    h[..., :3] = (h[..., 3:4] * rng.uniform(0.1, 1.0, size=(H, W, 3))).astype(np.float32)
+    return h
+
+
+@pytest.mark.parametrize("R,rmin,curve", [(9, 0, 0.4), (3, 1, 0.6), (0, 0, 0.4), (30, 0, 0.5)])
+def test_density_estimation_vs_oracle(rf, orc, gpu_available, R, rmin, curve):
+    import torch
+    rng = np.random.default_rng(R + 17)
+    W, H = 97, 75
+    h = _sparse_hist(rng, W, H)
+    ctx = _ctx(rf, 1 << 10, 1, rf.RF_ENGINE_FAST)
+    hist = ctx.histogram(W, H)
+    hist.upload(h)
+    out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    ctx.density_estimate(hist, rf.DEParams(R, rmin, curve), out.data_ptr())
+    torch.cuda.synchronize()
+    g = out.cpu().numpy()
+    ref = orc.density(h, R, rmin, curve, fp32=True)
+    tol = 2e-5 * np.abs(ref).max()
+    assert np.abs(g - ref).max() <= tol
+
+
+def test_density_identity_layout(rf, orc, gpu_available):
+    """R = 0: each bin lands on texel (px - 1, H - 1 - py); column 0 is clipped."""
+    import torch
+    W, H = 40, 30
+    h = np.zeros((H, W, 4), np.float32)
+    h[3, 7] = [1, 2, 3, 4]
+    h[10, 0] = [5, 5, 5, 5]
+    ctx = _ctx(rf, 1 << 10, 1, rf.RF_ENGINE_FAST)
+    hist = ctx.histogram(W, H)
+    hist.upload(h)
+    out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    ctx.density_estimate(hist, rf.DEParams(0, 0, 0.4), out.data_ptr())
+    g = out.cpu().numpy()
+    exp = np.zeros_like(g)
+    exp[H - 1 - 3, 7 - 1] = [1, 2, 3, 4]
+    assert np.array_equal(g, exp)
+
+
+def test_tonemap_vs_fp64_oracle(rf, orc, gpu_available):
+    import torch
+    rng = np.random.default_rng(8)
+    W, H = 64, 48
+    img = np.abs(rng.normal(size=(H, W, 4))).astype(np.float32) * 50
+    img[0, :5] = 0
+    ctx = _ctx(rf, 1 << 10, 1, rf.RF_ENGINE_FAST)
+    for gamma, bright, vib in [(2.2, 4.0, 1.0), (4.0, 29.7, 0.0), (1.5, 10.0, 0.5)]:
+        src = torch.from_numpy(img).cuda()
+        dst = torch.zeros_like(src)
+        ctx.tonemap(src.data_ptr(), dst.data_ptr(), W, H, rf.TonemapParams(gamma, bright, vib, 1e-4))
+        g = dst.cpu().numpy()
+        ref = orc.tonemap(img.astype(np.float64), gamma, bright, vib, 1e-4)
+        assert np.abs(g - ref).max() <= 2e-5
+
+
+def test_fused_density_tonemap_matches_split(rf, gpu_available):
+    import torch
+    rng = np.random.default_rng(2)
+    W, H = 80, 64
+    h = _sparse_hist(rng, W, H)
+    ctx = _ctx(rf, 1 << 10, 1, rf.RF_ENGINE_FAST)
+    hist = ctx.histogram(W, H)
+    hist.upload(h)
+    de = rf.DEParams(9, 0, 0.4)
+    tm = rf.TonemapParams(2.2, 4.0, 1.0, 1e-4)
+    a = torch.zeros((H, W, 4), device="cuda")
+    b = torch.zeros((H, W, 4), device="cuda")
+    fused = torch.zeros((H, W, 4), device="cuda")
+    ctx.density_estimate(hist, de, a.data_ptr())
+    ctx.tonemap(a.data_ptr(), b.data_ptr(), W, H, tm)
+    ctx.density_tonemap(hist, de, tm, fused.data_ptr())
+    assert torch.equal(b, fused)
+
+
+# -------------------------------------------------------- conservation, size
+@pytest.mark.parametrize("hist_mode", [1, 2])
+def test_contention_counts_conserved(rf, gpu_available, hist_mode):
+    """A strongly contractive flame: every binned sample adds exactly 1 to alpha."""
+    from refrakt_amd import configs
+    wl = configs.cfg3()
+    W = H = 1024
+    wl.flame.size = [W, H]
+    wl.flame.scale = W / 4.0
+    ctx = _ctx(rf, 1 << 18, 1, rf.RF_ENGINE_FAST, hist_mode=hist_mode)
+    fh = ctx.flame(wl.flame)
+    hist = ctx.histogram(W, H)
+    fh.warmup(19)
+    drawn = sum(fh.draw_to_bins(hist, 8) for _ in range(4))
+    g = hist.download()
+    assert g[..., 3].max() < 2 ** 24
+    assert drawn == (1 << 18) * 32
+    assert int(g[..., 3].astype(np.float64).sum()) == drawn
+
+
+def test_full_size_cfg2_conservation(rf, gpu_available):
+    """cfg2 at its real size (7680x4320, 2^24 particles, 2^32 binned samples)."""
+    from refrakt_amd import configs
+    wl = configs.cfg2()
+    ctx = _ctx(rf, wl.particles, wl.temporal_samples, rf.RF_ENGINE_FAST)
+    fh = ctx.flame(wl.flame)
+    hist = ctx.histogram(wl.width, wl.height)
+    fh.warmup(wl.warmup_passes)
+    drawn = sum(fh.draw_to_bins(hist, wl.draw_passes) for _ in range(wl.draw_calls))
+    g = hist.download()
+    assert np.isfinite(g).all()
+    assert g[..., 3].max() < 2 ** 24
+    assert 0.9 * wl.binned_iterations <= drawn <= wl.binned_iterations
+    assert int(g[..., 3].astype(np.float64).sum()) == drawn
+    pos = fh.particles()
+    assert np.isfinite(pos).all()

@@ -1,0 +1,73 @@
+"""Host-side logic that needs no GPU: parameter block, screen affine, validation."""
+import math
+
+import numpy as np
+import pytest
+
+
+def _flame(rf):
+    x0 = rf.Xform(affine=[1, 2, 3, 4, 5, 6], variations={"spherical": 0.25, "linear": 0.75}, weight=1.0,
+                  color=0.1, color_speed=0.2, opacity=0.9, rotation_frequency=1.0)
+    x1 = rf.Xform(affine=[7, 8, 9, 10, 11, 12], post=[0.5, 0, 0, 0.5, 1, -1],
+                  variations={"julian": 0.5, "blob": 0.5},
+                  var_param={"julian_power": 3.0, "julian_dist": 1.5, "blob_high": 2.0, "blob_low": 0.5,
+                             "blob_waves": 4.0},
+                  weight=3.0, color=0.7, color_speed=0.4, opacity=1.0)
+    fin = rf.Xform(affine=[1, 0, 0, 1, 0, 0], variations={"linear": 1.0}, weight=2.0, color=0.5, color_speed=0.0,
+                   opacity=1.0)
+    return rf.Flame(xforms=[x0, x1], final_xform=fin, size=[800, 600], center=[0.25, -0.5], scale=100.0,
+                    rotate=30.0)
+
+
+def test_param_block_layout(rf):
+    """make_shader_buffer_map + copy_flame_data_to_buffer (src/flame.cpp:33-103):
+    weight/sum, affine, [post], variation weights by name, params by name,
+    color, color_speed, opacity, rotation_frequency; final xform last."""
+    b = rf.param_block(_flame(rf))
+    f32 = np.float32
+    exp = [f32(1.0) / f32(4.0), 1, 2, 3, 4, 5, 6, 0.75, 0.25, 0.1, 0.2, f32(0.9), 1.0,
+           f32(3.0) / f32(4.0), 7, 8, 9, 10, 11, 12, 0.5, 0, 0, 0.5, 1, -1,
+           0.5, 0.5,                    # blob, julian (alphabetical)
+           2.0, 0.5, 4.0, 1.5, 3.0,     # blob_high, blob_low, blob_waves, julian_dist, julian_power
+           0.7, 0.4, 1.0, 0.0,
+           f32(2.0) / f32(4.0), 1, 0, 0, 1, 0, 0, 1.0, 0.5, 0.0, 1.0, 0.0]
+    assert b.tolist() == np.array(exp, np.float32).tolist()
+
+
+def test_screen_affine_closed_form(rf):
+    fl = _flame(rf)
+    W, H = 1600, 1200
+    ss = rf.screen_affine(fl, W, H).astype(np.float64)
+    s = 100.0 * H / 600.0
+    th = math.radians(30.0)
+    # rotate_affine (src/flame.hpp:97-109) on s*I: a' = s cos, b' = s sin, c' = -s sin, d' = s cos
+    a, b, c, d = s * math.cos(th), s * math.sin(th), -s * math.sin(th), s * math.cos(th)
+    # T(W/2,H/2) S(s) R(30 deg) T(-center): x' = a x + c y + e
+    e = W / 2 + a * -0.25 + c * 0.5
+    f = H / 2 + b * -0.25 + d * 0.5
+    np.testing.assert_allclose(ss, [a, b, c, d, e, f], rtol=1e-5, atol=1e-3)
+
+
+def test_validate_rejects_bad_flames(rf):
+    fl = _flame(rf)
+    assert rf.validate(fl) == ""
+    for x in fl.xforms:
+        x.weight = 0.0
+    assert "weights" in rf.validate(fl)
+    fl2 = _flame(rf)
+    fl2.xforms = []
+    with pytest.raises(rf.RefraktError):
+        fl2.to_c()
+
+
+def test_configs_are_deterministic(rf):
+    from refrakt_amd import configs
+    a, b = configs.cfg2(), configs.cfg2()
+    assert rf.param_block(a.flame).tolist() == rf.param_block(b.flame).tolist()
+    assert a.binned_iterations == 1 << 32
+    assert configs.cfg1().binned_iterations == 1 << 30
+    assert configs.cfg3().binned_iterations == 1 << 34
+    c2 = a.flame
+    assert len(c2.xforms) == 8 and c2.final_xform is not None
+    assert all(x.post is not None and 3 <= len(x.variations) <= 5 for x in c2.xforms)
+    assert all(abs(sum(x.variations.values()) - 1) < 1e-6 for x in c2.xforms)
