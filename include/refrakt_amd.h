@@ -174,7 +174,7 @@ void rf_destroy(rf_ctx* ctx);
 int rf_set_sim_parameters(rf_ctx* ctx, uint64_t total_particles,
                           uint32_t temporal_samples, uint32_t shuffle_count);
 int rf_set_engine(rf_ctx* ctx, int32_t engine, int32_t hist_mode);
-const char* rf_last_error(const rf_ctx* ctx);
+const char* rf_last_error(const rf_ctx* ctx);  /* NULL: why the last rf_create failed */
 /* Device time (ms, HIP events on the context stream) of the chaos-game
  * launches of the last rf_draw_to_bins call. */
 double rf_last_draw_ms(const rf_ctx* ctx);

@@ -140,6 +140,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RefraktError(f"{LIB_PATH} is missing: run refrakt_amd.build (or __graft_entry__.build())")
+        # torch ships its own HIP runtime under the file name libamdhip64.so
+        # (soname libamdhip64.so.7).  Loading torch first makes this library's
+        # libamdhip64.so.7 dependency resolve to that same runtime; the other
+        # order puts two HIP runtimes in one process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)
@@ -334,7 +342,7 @@ class Context:
         h = C.c_void_p()
         rc = self.L.rf_create(C.byref(sp), C.byref(h))
         if rc != RF_OK:
-            raise RefraktError(f"rf_create failed ({rc})")
+            raise RefraktError(f"rf_create failed ({rc}): {self.L.rf_last_error(None).decode()}")
         self.h = h
         self.total_particles = total_particles
         self.temporal_samples = temporal_samples

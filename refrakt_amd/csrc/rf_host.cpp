@@ -478,6 +478,9 @@ static rfk::DEArgs de_args(rf_hist* h, const rf_de_params* de) {
   return a;
 }
 
+// Reason of the last failed rf_create (rf_last_error(NULL)).
+static thread_local std::string g_create_err;
+
 // ================================================================== C ABI
 extern "C" {
 
@@ -499,7 +502,7 @@ int rf_variation_flags(int id) {
   return v ? (int)(v->flags | (v->uses_rand ? 0x100u : 0u)) : -1;
 }
 
-const char* rf_last_error(const rf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* rf_last_error(const rf_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 double rf_last_draw_ms(const rf_ctx* ctx) { return ctx ? ctx->last_draw_ms : -1.0; }
 void* rf_stream(rf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
@@ -510,10 +513,13 @@ int rf_create(const rf_sim_params* p, rf_ctx** out) {
   ctx->sp = *p;
   if (ctx->sp.shuffle_count == 0) ctx->sp.shuffle_count = 1024;
   int rc = validate_sim(ctx, p->total_particles, p->temporal_samples);
-  if (rc) { delete ctx; return rc; }
-  if (hipSetDevice(p->device) != hipSuccess) { delete ctx; return RF_ERR_HIP; }
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return RF_ERR_HIP; }
-  if (ctx->counter.alloc(8) != hipSuccess) { rf_destroy(ctx); return RF_ERR_HIP; }
+  if (rc) { g_create_err = ctx->err; delete ctx; return rc; }
+  hipError_t e = hipSetDevice(p->device);
+  if (e != hipSuccess) { g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e); delete ctx; return RF_ERR_HIP; }
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e); delete ctx; return RF_ERR_HIP; }
+  e = ctx->counter.alloc(8);
+  if (e != hipSuccess) { g_create_err = std::string("hipMalloc: ") + hipGetErrorString(e); rf_destroy(ctx); return RF_ERR_HIP; }
   ctx->ppt = p->total_particles / p->temporal_samples;
   ctx->shuffle_count = ctx->sp.shuffle_count;
   ctx->row_state = p->seed ^ 0x0123456789ABCDEFull;

@@ -100,7 +100,9 @@ __device__ __forceinline__ float de_weight(const DESrc& m, int i, int j) {
 // where the shader divides 0/0) maps to black.
 __device__ __forceinline__ float4 tonemap_px(float4 c, float gamma, float brightness, float vib, float k) {
   if (!(c.w > 0.0f)) return make_float4(0.f, 0.f, 0.f, 1.0f);
-  const float ls = (((0.5f * brightness) * logf(1.0f + c.w * k)) * LOG10E) / c.w;
+  // log(1 + a k) through log1pf: the shader's float log(1.0 + a*k) loses most
+  // digits when a*k is small (k = 1e-4 makes that every sparse texel).
+  const float ls = (((0.5f * brightness) * log1pf(c.w * k)) * LOG10E) / c.w;
   c.x *= ls; c.y *= ls; c.z *= ls; c.w *= ls;
   const float ig = 1.0f / gamma;
   const float z = powf(c.w, ig);
@@ -220,9 +222,9 @@ hipError_t launch_animate(Block* blocks, uint32_t T, float tss_width, const floa
 }
 
 hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s) {
-  if (a.R <= 24) {
-    const int SW = DE_TW + 2 * a.R, SH = DE_TH + 2 * a.R;
-    const size_t sm = (size_t)SW * SH * (sizeof(float4) + sizeof(DESrc));
+  const int SW = DE_TW + 2 * a.R, SH = DE_TH + 2 * a.R;
+  const size_t sm = (size_t)SW * SH * (sizeof(float4) + sizeof(DESrc));
+  if (sm <= 65536) {  // within the default dynamic-LDS limit of one launch (R <= 13)
     dim3 grid((a.W + DE_TW - 1) / DE_TW, (a.H + DE_TH - 1) / DE_TH);
     hipLaunchKernelGGL(k_de_tiled, grid, dim3(DE_TW * DE_TH), sm, s, a);
   } else {
