@@ -19,7 +19,8 @@
  *   - Device memory is owned by the library.  Host arrays passed in are owned
  *     by the caller and only read or written during the call.
  *   - Calls on one rf_ctx are serialised by the caller; distinct contexts are
- *     independent.  Each context owns one HIP stream.
+ *     independent.  Each context owns one HIP stream, created blocking so it
+ *     orders with the legacy default stream (where torch and hipMemcpy run).
  *   - Histograms are W*H RGBA float32 bins, row-major, rows stored bottom-up
  *     exactly like the reference's `bins` SSBO (shaders/flame.glsl:84):
  *     screen pixel (x, y) lives at bin (H-1-y)*W + x.
@@ -251,7 +252,7 @@ int rf_particles_download(rf_ctx* ctx, rf_flame* flame, float* host);
 int rf_rng_download(rf_ctx* ctx, uint32_t* host);
 int rf_shuffle_download(rf_ctx* ctx, uint32_t* host);
 int rf_sample_points_download(rf_ctx* ctx, float* host);
-/* Fast engine: the 256 LDS permutation rows (256 x 64*block_waves u16). */
+/* Fast engine: the 16 LDS permutation rows (16 x 64*block_waves u16). */
 int rf_fast_perms_download(rf_ctx* ctx, uint16_t* host);
 /* Screen-space affine used by draw_to_bins (src/flame.cpp:289-296). */
 int rf_screen_affine(const rf_flame_desc* desc, uint32_t width, uint32_t height, float out[6]);

@@ -88,6 +88,55 @@ __host__ __device__ __forceinline__ void rf_sincos01(float q, float* s, float* c
   *c = fmaf(pc, z, 1.0f);
 }
 
+// ------------------------------------------------------------------ math
+// Transcendentals of the variation interpreter.  The device library's sinf
+// and friends inline a Payne-Hanek reduction at every call site, and the
+// compiler if-converts it, so every call paid for both paths; in a kernel with
+// ~100 trig call sites that multiplied the iteration's VALU work ~8x.
+//
+// sin/cos: Cody-Waite reduction by pi/2 with a three-part constant, then the
+// minimax polynomials of the Cephes library's sinf/cosf (S. L. Moshier) on
+// [-pi/4, pi/4] -- the same method Mesa's llvmpipe uses for GLSL sin/cos.
+// Accurate to a few ulp for |x| < ~1e5 (the j*C1 product is exact below 2^16).
+__device__ __forceinline__ void rf_sincos(float x, float* s, float* c) {
+  const float j = rintf(x * 0.636619772367581343f);
+  float r = fmaf(-j, 1.5703125f, x);
+  r = fmaf(-j, 4.837512969970703125e-4f, r);
+  r = fmaf(-j, 7.54978995489188216e-8f, r);
+  const float z = r * r;
+  float ps = fmaf(fmaf(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f);
+  ps = fmaf(ps * z, r, r);
+  float pc = fmaf(fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z, 4.166664568298827e-2f);
+  pc = fmaf(pc * z, z, fmaf(-0.5f, z, 1.0f));
+  const int q = (int)j;
+  float sn = (q & 1) ? pc : ps;
+  float cs = (q & 1) ? ps : pc;
+  if (q & 2) sn = -sn;
+  if ((q + 1) & 2) cs = -cs;
+  *s = sn;
+  *c = cs;
+}
+__device__ __forceinline__ float rf_sin(float x) { float s, c; rf_sincos(x, &s, &c); return s; }
+__device__ __forceinline__ float rf_cos(float x) { float s, c; rf_sincos(x, &s, &c); return c; }
+__device__ __forceinline__ float rf_tan(float x) { float s, c; rf_sincos(x, &s, &c); return s / c; }
+// exp, log and pow through the hardware base-2 instructions; GLSL defines
+// pow(x, y) as exp2(y * log2(x)) for x > 0.
+__device__ __forceinline__ float rf_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float rf_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float rf_exp(float x) { return rf_exp2(x * 1.44269504088896341f); }
+__device__ __forceinline__ float rf_log(float x) { return rf_log2(x) * 0.693147180559945309f; }
+__device__ __forceinline__ float rf_log10(float x) { return rf_log2(x) * 0.301029995663981195f; }
+__device__ __forceinline__ float rf_pow(float x, float y) { return rf_exp2(y * rf_log2(x)); }
+__device__ __forceinline__ float rf_sinh(float x) { const float e = rf_exp(x); return 0.5f * (e - 1.0f / e); }
+__device__ __forceinline__ float rf_cosh(float x) { const float e = rf_exp(x); return 0.5f * (e + 1.0f / e); }
+// Correctly rounded sqrt and division for the few places whose result must
+// match the CPU bit for bit (the library is built with fast fp32 div/sqrt).
+// They go through binary64: rounding the correctly rounded double result to
+// float is exact-rounding for sqrt and division because 53 >= 2*24 + 2
+// (Figueroa, "When is double rounding innocuous?", SIGNUM 1995).
+__device__ __forceinline__ float rf_sqrt_rn(float x) { return (float)sqrt((double)x); }
+__device__ __forceinline__ float rf_div_rn(float x, float y) { return (float)((double)x / (double)y); }
+
 // Palette index min(255, uint(ceil(c * 255))) (shaders/flame.glsl:84), with
 // negative and NaN colours mapped to 0 (undefined in GLSL).
 __host__ __device__ __forceinline__ uint32_t rf_pal_index(float c) {
@@ -237,8 +286,7 @@ __device__ __forceinline__ XOut apply_xform(const Block* __restrict__ blk, int x
   if (m & PC_PHI) pc.phi = atan2f(y, x);
   if (m & PC_SINA) pc.sina = x / pc.r;
   if (m & PC_COSA) pc.cosa = y / pc.r;
-  if (m & PC_SINR) pc.sinr = sinf(pc.r);
-  if (m & PC_COSR) pc.cosr = cosf(pc.r);
+  if (m & (PC_SINR | PC_COSR)) rf_sincos(pc.r, &pc.sinr, &pc.cosr);
   float rx = 0.0f, ry = 0.0f;
   for (uint32_t k = npre; k < nt; ++k) {
     float P[MAX_PRM];

@@ -516,7 +516,10 @@ int rf_create(const rf_sim_params* p, rf_ctx** out) {
   if (rc) { g_create_err = ctx->err; delete ctx; return rc; }
   hipError_t e = hipSetDevice(p->device);
   if (e != hipSuccess) { g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e); delete ctx; return RF_ERR_HIP; }
-  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  // A blocking stream: it orders with the legacy default stream, so device
+  // buffers that a caller (e.g. torch) filled there are ready before our
+  // kernels read or write them, and the caller's later work sees our results.
+  e = hipStreamCreate(&ctx->stream);
   if (e != hipSuccess) { g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e); delete ctx; return RF_ERR_HIP; }
   e = ctx->counter.alloc(8);
   if (e != hipSuccess) { g_create_err = std::string("hipMalloc: ") + hipGetErrorString(e); rf_destroy(ctx); return RF_ERR_HIP; }

@@ -33,7 +33,7 @@ struct FastArgs {
   const float4* palette;
   float* bins;
   unsigned long long* counter;
-  const uint16_t* perms;         // NPERM rows x (64*waves) slot permutation
+  const uint16_t* perms;         // NPERM rows x (64*waves) slot permutations
   uint64_t seed;
   uint32_t launch;               // launch counter (RNG streams)
   uint32_t ppt;
@@ -54,7 +54,7 @@ struct DEArgs {
   float gamma, brightness, vibrancy, k;
 };
 
-constexpr int NPERM = 256;
+constexpr int NPERM = 16;         // permutation rows of the fast engine (held in LDS)
 
 hipError_t launch_init_rs(uint4* rs, uint64_t n, hipStream_t s);
 hipError_t launch_compat_pass(const CompatArgs& a, int hist_mode, hipStream_t s);

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_compat_pass(CompatArgs g) {
     const float r2 = rng.randf();
     const float s = ((r1 * 0.1f) * PI) * 2.0f;
     float sn, cs;
-    rf_sincos01(sqrtf(r2), &sn, &cs);
+    rf_sincos01(rf_sqrt_rn(r2), &sn, &cs);
     p.x = p.x + s * sn;
     p.y = p.y + s * cs;
   }

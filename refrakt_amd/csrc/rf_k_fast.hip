@@ -21,14 +21,21 @@ using namespace detail;
 template <int WAVES, int HM>
 __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g) {
   constexpr int N = 64 * WAVES;
-  __shared__ float s_x[N], s_y[N], s_c[N];
+  // Double-buffered exchange: iteration `it` writes buffer it&1, so one
+  // barrier per iteration separates every write from the reads it must not
+  // overtake (the next write to the same buffer is two barriers later).
+  __shared__ float s_x[2][N], s_y[2][N], s_c[2][N];
   __shared__ float4 s_pal[256];
+  // The permutation rows live in LDS: a global load inside the loop would sit
+  // behind the iteration's atomics in the in-order vmcnt queue and stall.
+  __shared__ uint16_t s_perm[NPERM * N];
   const uint32_t tid = threadIdx.x;
   const uint32_t base = blockIdx.x * (uint32_t)N;
   const uint32_t pid = base + tid;
   const uint32_t t = base / g.ppt;
   const Block* __restrict__ blk = g.blocks + (size_t)t * g.block_stride;
   for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palette[i];
+  for (uint32_t i = tid; i < (uint32_t)(NPERM * N); i += N) s_perm[i] = g.perms[i];
 
   Jsf32 lrng = jsf_from_philox(g.seed, pid, g.launch, 0u);
   const uint32_t wid = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + (tid >> 6));
@@ -41,7 +48,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g) {
     // like the first run of shaders/flame.glsl:63-66.
     const uint32_t cnt = g.ppt;
     const uint32_t mx = (cnt % 2u == 0u) ? cnt : (1u << (32 - __clz(cnt - 1u)));
-    const float inv_max = 1.0f / (float)mx;
+    const float inv_max = rf_div_rn(1.0f, (float)mx);
     const uint32_t rbits = 31u - __clz(mx);
     const uint32_t i = pid % cnt;
     const uint32_t fl = __brev(i) >> (32u - rbits);
@@ -52,7 +59,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g) {
     const float r2 = lrng.randf();
     const float s = ((r1 * 0.1f) * PI) * 2.0f;
     float sn, cs;
-    rf_sincos01(sqrtf(r2), &sn, &cs);
+    rf_sincos01(rf_sqrt_rn(r2), &sn, &cs);
     x = x + s * sn;
     y = y + s * cs;
   } else {
@@ -80,11 +87,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g) {
       drawn += (unsigned long long)__popcll(__ballot(v));
     }
     const uint32_t row = grng.next_a() & (uint32_t)(NPERM - 1);
-    const uint32_t dst = g.perms[(size_t)row * N + tid];
-    s_x[dst] = x; s_y[dst] = y; s_c[dst] = c;
-    __syncthreads();
-    x = s_x[tid]; y = s_y[tid]; c = s_c[tid];
-    __syncthreads();
+    const uint32_t dst = s_perm[row * N + tid];
+    const uint32_t b = it & 1u;
+    s_x[b][dst] = x; s_y[b][dst] = y; s_c[b][dst] = c;
+    lds_barrier();
+    x = s_x[b][tid]; y = s_y[b][tid]; c = s_c[b][tid];
   }
   g.pos[pid] = make_float4(x, y, c, 0.0f);
   if (g.do_draw) count_drawn(g.counter, drawn);

@@ -72,7 +72,7 @@ __device__ __forceinline__ void de_source(const DEArgs& g, int cx, int cy, float
   if (cx < 0 || px >= (int)g.W || cy < 0 || cy >= (int)g.H) return;
   const float4 b = g.hist[(size_t)py * g.W + px];
   if (!(b.w != 0.0f)) return;  // color.a == 0.0 -> point moved out of the view volume
-  const float q = (float)g.R / powf(b.w, g.curve);
+  const float q = rf_div_rn((float)g.R, powf(b.w, g.curve));
   int ri = (q >= (float)g.R) ? g.R : (int)q;
   if (!(q == q)) ri = g.R;
   int r = ri > g.rmin ? ri : g.rmin;
@@ -80,8 +80,8 @@ __device__ __forceinline__ void de_source(const DEArgs& g, int cx, int cy, float
   col = b;
   m.r = (float)r;
   const int S = 2 * r + 1;
-  m.e = 1.0f / (float)S;
-  m.nrm = (r > 0) ? 0.63661977236f / (float)(r * r) : 1.0f;
+  m.e = rf_div_rn(1.0f, (float)S);
+  m.nrm = (r > 0) ? rf_div_rn(0.63661977236f, (float)(r * r)) : 1.0f;
 }
 
 __device__ __forceinline__ float de_weight(const DESrc& m, int i, int j) {
@@ -102,11 +102,11 @@ __device__ __forceinline__ float4 tonemap_px(float4 c, float gamma, float bright
   if (!(c.w > 0.0f)) return make_float4(0.f, 0.f, 0.f, 1.0f);
   // log(1 + a k) through log1pf: the shader's float log(1.0 + a*k) loses most
   // digits when a*k is small (k = 1e-4 makes that every sparse texel).
-  const float ls = (((0.5f * brightness) * log1pf(c.w * k)) * LOG10E) / c.w;
+  const float ls = rf_div_rn(((0.5f * brightness) * log1pf(c.w * k)) * LOG10E, c.w);
   c.x *= ls; c.y *= ls; c.z *= ls; c.w *= ls;
-  const float ig = 1.0f / gamma;
+  const float ig = rf_div_rn(1.0f, gamma);
   const float z = powf(c.w, ig);
-  const float gf = z / c.w;
+  const float gf = rf_div_rn(z, c.w);
   float o[3] = {c.x, c.y, c.z};
 #pragma unroll
   for (int q = 0; q < 3; ++q) {

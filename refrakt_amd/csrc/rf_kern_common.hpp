@@ -131,6 +131,15 @@ __device__ __forceinline__ bool draw_point(const Block* __restrict__ blk, const 
   return valid;
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for
+// every outstanding global access (vmcnt(0)), which would stall each
+// iteration on the round trip of its fire-and-forget histogram atomics.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ void count_drawn(unsigned long long* counter, unsigned long long n) {
   const int lane = threadIdx.x & (WAVE - 1);
   if (lane == 0 && n) atomicAdd(counter, n);
