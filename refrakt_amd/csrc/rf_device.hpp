@@ -55,9 +55,21 @@ struct Block {
   uint32_t n_terms;
   uint32_t pad;
   float cdf[MAX_XF];     // running sum of normalised weights, in xform order
+  // For a raw 32-bit draw x with u = float(x) * 2^-32:  cdf[i] >= u  <=>
+  // (never_mask bit i clear and x <= cdf_lim[i]).  Lets a wave select on SALU.
+  uint32_t cdf_lim[MAX_XF];
+  uint32_t never_mask;
+  uint32_t pad2[3];
   Xform xf[MAX_XF + 1];
   Term terms[MAX_TERMS];
 };
+
+// A Block seen through the constant address space: it is never written while
+// a chaos-game kernel runs, and loads through such a pointer are always
+// scalar (s_load) when their address is uniform, so the flame is read through
+// the scalar cache instead of vector loads that would queue behind atomics.
+typedef const __attribute__((address_space(4))) Block CBlock;
+__device__ __forceinline__ CBlock* as_constant(const Block* p) { return (CBlock*)p; }
 
 // ---------------------------------------------------------------- helpers
 __host__ __device__ __forceinline__ float rf_mod2(float x, float y) {
@@ -255,17 +267,17 @@ struct XOut { float x, y, c, w; };
 // Evaluation order follows variation_table.cpp:97-165: fma affine, pre_xform
 // terms added into v, precalcs, the regular terms summed in alphabetical order
 // (separate multiply and add), then the fma post-affine.
-template <class Rng>
-__device__ __forceinline__ XOut apply_xform(const Block* __restrict__ blk, int xi, float x0, float y0,
+template <class Rng, class BP>
+__device__ __forceinline__ XOut apply_xform(BP blk, int xi, float x0, float y0,
                                             float c0, bool first, Rng& rng) {
-  const Xform* __restrict__ xf = &blk->xf[xi];
+  const auto* __restrict__ xf = &blk->xf[xi];
   float A[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) A[k] = xf->aff[k];
   float x = fmaf(A[0], x0, fmaf(A[2], y0, A[4]));
   float y = fmaf(A[1], x0, fmaf(A[3], y0, A[5]));
   const uint32_t tb = xf->term_begin, npre = xf->n_pre, nt = xf->n_terms;
-  const Term* __restrict__ T = blk->terms + tb;
+  const auto* __restrict__ T = blk->terms + tb;
   Pre pc;
   pc.r = pc.rsq = pc.a = pc.phi = pc.sina = pc.cosa = pc.sinr = pc.cosr = 0.0f;
   for (uint32_t k = 0; k < npre; ++k) {
@@ -293,8 +305,20 @@ __device__ __forceinline__ XOut apply_xform(const Block* __restrict__ blk, int x
 #pragma unroll
     for (int j = 0; j < MAX_PRM; ++j) P[j] = T[k].prm[j];
     const float w = T[k].weight;
+    // Everything a variation computes from x, y, the precalcs or the affine is
+    // invariant in this loop, and loop-invariant code motion would hoist it out
+    // of the switch -- i.e. evaluate sin(x), sin(rsq), exp(x - 1), ... of all 76
+    // variations on every xform application.  An empty asm that claims to
+    // rewrite these registers keeps each case's work inside its case.
+    float xk = x, yk = y, Ak[6];
+    Pre pk = pc;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) Ak[j] = A[j];
+    asm volatile("" : "+v"(xk), "+v"(yk), "+v"(pk.r), "+v"(pk.rsq), "+v"(pk.a), "+v"(pk.phi), "+v"(pk.sina),
+                 "+v"(pk.cosa), "+v"(pk.sinr), "+v"(pk.cosr));
+    asm volatile("" : "+s"(Ak[0]), "+s"(Ak[1]), "+s"(Ak[2]), "+s"(Ak[3]), "+s"(Ak[4]), "+s"(Ak[5]));
     float ox, oy;
-    eval_var(T[k].var, x, y, pc, w, A, P, rng, ox, oy);
+    eval_var(T[k].var, xk, yk, pk, w, Ak, P, rng, ox, oy);
     if (!(T[k].flags & TF_NO_WEIGHT_MUL)) { ox = w * ox; oy = w * oy; }
     if (k == npre) { rx = ox; ry = oy; }
     else { rx = rx + ox; ry = ry + oy; }
@@ -316,10 +340,24 @@ __device__ __forceinline__ XOut apply_xform(const Block* __restrict__ blk, int x
 
 // get_xform_id (shaders/templates/xform_select.tpl.glsl:1-14): the first i
 // whose running weight sum reaches u; the last xform is unconditional.
-__device__ __forceinline__ int select_xform(const Block* __restrict__ blk, float u) {
+template <class BP>
+__device__ __forceinline__ int select_xform(BP blk, float u) {
   const int n = (int)blk->n_xforms;
   int i = 0;
   while (i < n - 1 && !(blk->cdf[i] >= u)) ++i;
+  return i;
+}
+
+// The same selection for a wave-uniform raw draw x (u = float(x) * 2^-32),
+// entirely on the scalar unit: gfx950 has no scalar float compare, so the
+// host precomputes integer limits with cdf[i] >= u <=> x <= cdf_lim[i].
+template <class BP>
+__device__ __forceinline__ int select_xform_raw(BP blk, uint32_t x_any) {
+  const uint32_t x = __builtin_amdgcn_readfirstlane(x_any);
+  const int n = (int)blk->n_xforms;
+  const uint32_t never = blk->never_mask;
+  int i = 0;
+  while (i < n - 1 && !(((never >> i) & 1u) == 0u && x <= blk->cdf_lim[i])) ++i;
   return i;
 }
 

@@ -112,6 +112,22 @@ bool build_block(const rf_flame_desc& d, Block& b, std::string& err) {
     const float wn = d.xforms[i].weight / total;
     sum = (i == 0) ? wn : sum + wn;
     b.cdf[i] = sum;
+    // Largest raw draw x with float(x) * 2^-32 <= cdf[i] (the float
+    // conversion rounds to nearest even on host and device alike, and is
+    // monotone in x, so a binary search finds it).
+    auto u_of = [](uint32_t x) { return (float)x * 0x1p-32f; };
+    if (!(sum >= 0.0f)) {
+      b.never_mask |= 1u << i;
+    } else if (u_of(0xffffffffu) <= sum) {
+      b.cdf_lim[i] = 0xffffffffu;
+    } else {
+      uint32_t lo = 0, hi = 0xffffffffu;  // u_of(lo) <= sum < u_of(hi)
+      while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (u_of(mid) <= sum) lo = mid; else hi = mid;
+      }
+      b.cdf_lim[i] = lo;
+    }
   }
   uint32_t nt = 0;
   auto put = [&](const rf_xform& x, rfd::Xform& X) -> bool {

@@ -16,7 +16,7 @@ using namespace detail;
 // xform drawn by lane 0 for the whole workgroup, per-slot jsf32 state,
 // shuffle-row gather on read (and scatter on write during warmup).
 template <int HM>
-__global__ __launch_bounds__(256) void k_compat_pass(CompatArgs g) {
+__global__ __launch_bounds__(256) void k_compat_pass(CompatArgs g, const Block* __restrict__ blocks) {
   __shared__ int s_xid;
   __shared__ float4 s_pal[256];
   const uint32_t gx = blockIdx.x * 256u + threadIdx.x;
@@ -24,7 +24,7 @@ __global__ __launch_bounds__(256) void k_compat_pass(CompatArgs g) {
   const size_t slot = (size_t)y * g.ppt + gx;
   const uint4 st = g.rs[slot];
   Jsf32 rng{st.x, st.y, st.z, st.w};
-  const Block* __restrict__ blk = g.blocks + (size_t)y * g.block_stride;
+  CBlock* blk = as_constant(blocks + (size_t)y * g.block_stride);
   if (g.do_draw) s_pal[threadIdx.x] = g.palette[threadIdx.x];
   if (threadIdx.x == 0) s_xid = select_xform(blk, rng.randf());
   const uint32_t i_idx = g.random_read ? g.shuf[(size_t)g.in_row * g.ppt + gx] : gx;
@@ -70,9 +70,9 @@ hipError_t launch_init_rs(uint4* rs, uint64_t n, hipStream_t s) {
 hipError_t launch_compat_pass(const CompatArgs& a, int hist_mode, hipStream_t s) {
   dim3 grid(a.ppt / 256u, a.T), block(256);
   switch (hist_mode) {
-    case 2: hipLaunchKernelGGL(k_compat_pass<2>, grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(k_compat_pass<3>, grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL(k_compat_pass<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_compat_pass<2>, grid, block, 0, s, a, a.blocks); break;
+    case 3: hipLaunchKernelGGL(k_compat_pass<3>, grid, block, 0, s, a, a.blocks); break;
+    default: hipLaunchKernelGGL(k_compat_pass<1>, grid, block, 0, s, a, a.blocks); break;
   }
   return hipGetLastError();
 }

@@ -19,7 +19,7 @@ using namespace detail;
 // which decorrelates the xform sequences of particles sharing a wave (the
 // role of the reference's shuffle rows, src/shuffle_buffers.cpp).
 template <int WAVES, int HM>
-__global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g) {
+__global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __restrict__ blocks) {
   constexpr int N = 64 * WAVES;
   // Double-buffered exchange: iteration `it` writes buffer it&1, so one
   // barrier per iteration separates every write from the reads it must not
@@ -32,8 +32,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g) {
   const uint32_t tid = threadIdx.x;
   const uint32_t base = blockIdx.x * (uint32_t)N;
   const uint32_t pid = base + tid;
-  const uint32_t t = base / g.ppt;
-  const Block* __restrict__ blk = g.blocks + (size_t)t * g.block_stride;
+  // Integer division runs on the VALU, so the quotient is not known to be
+  // uniform; without readfirstlane every Block access would be a vector load.
+  const uint32_t t = __builtin_amdgcn_readfirstlane(base / g.ppt);
+  CBlock* blk = as_constant(blocks + (size_t)t * g.block_stride);
   for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palette[i];
   for (uint32_t i = tid; i < (uint32_t)(NPERM * N); i += N) s_perm[i] = g.perms[i];
 
@@ -70,8 +72,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g) {
 
   unsigned long long drawn = 0;
   for (uint32_t it = 0; it < g.n_iter; ++it) {
-    const float u = wrng.randf();
-    const int xid = select_xform(blk, u);
+    // The draw is wave-uniform; selecting on the raw integer keeps xid in SGPRs
+    // (float compares would run on the VALU, make xid look divergent and turn
+    // every xform/term access into vector loads behind the atomics).
+    const int xid = select_xform_raw(blk, wrng.next_a());
     XOut o = apply_xform(blk, xid, x, y, c, g.first_run && it == 0, lrng);
     if (rf_badval(o.x) || rf_badval(o.y)) {
       // A particle that left the finite plane restarts at a random point and
@@ -103,9 +107,9 @@ template <int W>
 static void launch_fast_w(const FastArgs& a, uint64_t n, int hm, hipStream_t s) {
   dim3 grid((uint32_t)(n / (64u * W))), block(64 * W);
   switch (hm) {
-    case 2: hipLaunchKernelGGL((k_fast<W, 2>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_fast<W, 3>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_fast<W, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_fast<W, 2>), grid, block, 0, s, a, a.blocks); break;
+    case 3: hipLaunchKernelGGL((k_fast<W, 3>), grid, block, 0, s, a, a.blocks); break;
+    default: hipLaunchKernelGGL((k_fast<W, 1>), grid, block, 0, s, a, a.blocks); break;
   }
 }
 

@@ -110,8 +110,8 @@ __device__ __forceinline__ void bin_sample(float* __restrict__ bins, uint32_t bi
 // add (shaders/flame.glsl:74-87).  Float comparisons replace the integer
 // ones of the reference: floor(s) >= 0 <=> s >= 0 and floor(s) < W <=> s < W,
 // and NaN fails both (the reference's int conversion of NaN is undefined).
-template <int HM, class Rng>
-__device__ __forceinline__ bool draw_point(const Block* __restrict__ blk, const XOut& o, Rng& rng,
+template <int HM, class Rng, class BP>
+__device__ __forceinline__ bool draw_point(BP blk, const XOut& o, Rng& rng,
                                            const float ss[6], uint32_t W, uint32_t H,
                                            float* __restrict__ bins, const float4* __restrict__ pal) {
   float rx = o.x, ry = o.y, rc = o.c, rw = o.w;
