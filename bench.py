@@ -20,6 +20,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+HBM_PEAK_GBS = 8000.0
+# HBM bytes per draw launch from rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
+# (scale factors from a 512 MiB copy), profiles/k_fast_cfg2_hbm_counters.md.
+# Keyed by (config, hist mode, waves per block); other keys report null.
+MEASURED_TRAFFIC = {("cfg2", 1, 4): 9.06e9}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -32,6 +38,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-probe", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 512 MiB device-to-device copy (HBM counter calibration) and exit")
     return ap.parse_args()
 
 
@@ -40,18 +48,16 @@ def cpu_baseline(wl, budget_s):
     from oracle import pyoracle as po
     threads = max(1, min(16, os.cpu_count() or 1))
     W, H = wl.width, wl.height
-    # calibrate on a small run, then size the sample to ~budget_s seconds
-    parts, iters, fuse = 1 << 12, 64, 20
-    t0 = time.perf_counter()
-    _, n0 = po.chaos_f64(wl.flame, W // 8, H // 8, parts, fuse, iters, seed=3, threads=threads)
-    dt0 = time.perf_counter() - t0
-    rate0 = (parts * (iters + fuse)) / max(dt0, 1e-6)
-    target = max(1, int(rate0 * budget_s * 0.8))
-    iters = 256
-    parts = max(threads * 64, target // (iters + fuse))
-    t0 = time.perf_counter()
-    _, n = po.chaos_f64(wl.flame, W // 4, H // 4, parts, fuse, iters, seed=4, threads=threads)
-    dt = time.perf_counter() - t0
+    # grow the sample until one run takes about budget_s seconds
+    fuse, iters = 20, 256
+    parts = threads * 256
+    while True:
+        t0 = time.perf_counter()
+        _, n = po.chaos_f64(wl.flame, W // 4, H // 4, parts, fuse, iters, seed=4, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 * budget_s or parts >= (1 << 24):
+            break
+        parts = int(parts * min(16.0, max(2.0, budget_s / max(dt, 1e-3))))
     return {"value": n / dt, "unit": "binned samples/s", "cores": threads, "kind": "port",
             "sample": f"{wl.name} flame, FP64 C oracle (per-particle chaos game), {parts} particles x "
                       f"({fuse} unbinned + {iters} binned) iterations on {threads} threads into a "
@@ -65,6 +71,14 @@ def main():
                                    "design note, see DESIGN.md)"}))
         sys.exit(2)
     import torch
+    if args.copy_calibration:
+        src = torch.empty(1 << 27, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "launches": 3}))
+        return
     import refrakt_amd as rf
     from refrakt_amd import configs
 
@@ -100,13 +114,23 @@ def main():
         probe = samples / (ms / 1e3)
     k_avg = float(np.mean(kernel_ms))
     per_launch_samples = drawn / args.steps
-    achieved = per_launch_samples * 4 / (k_avg / 1e3) / 1e9          # G atomic dword-adds/s
-    peak = probe * 4 / 1e9 if probe else None
-    roofline = {"bound": "l2_atomic", "achieved": round(achieved, 2), "peak": round(peak, 2) if peak else None,
-                "unit": "G atomic f32 adds/s", "frac": round(achieved / peak, 4) if peak else None,
-                "traffic": None, "kernel": "k_fast", "kernel_avg_ms": round(k_avg, 3),
-                "note": "one binned sample = one RGBA FP32 add = 4 f32 atomics in one 16-byte L2 request; "
-                        "peak = live probe of the same request shape on a 1 MiB footprint"}
+    # Algorithmic HBM bytes of one draw launch: a binned sample reads and
+    # writes one 16-byte RGBA bin; each particle's (x, y, color) is read and
+    # written once per launch.
+    alg_bytes = per_launch_samples * 32 + wl.particles * 24
+    hbm_gbs = alg_bytes / (k_avg / 1e3) / 1e9
+    atomic_g = per_launch_samples * 4 / (k_avg / 1e3) / 1e9          # G f32 atomic adds/s
+    peak_atomic = probe * 4 / 1e9 if probe else None
+    roofline = {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
+                "traffic": MEASURED_TRAFFIC.get((args.config, args.hist_mode, args.waves)),
+                "kernel": "k_fast", "kernel_avg_ms": round(k_avg, 3),
+                "atomic": {"achieved": round(atomic_g, 2), "peak": round(peak_atomic, 2) if peak_atomic else None,
+                           "unit": "G f32 atomic adds/s",
+                           "frac": round(atomic_g / peak_atomic, 4) if peak_atomic else None},
+                "note": "algorithmic bytes = 32 B per binned sample (16 B RGBA read + write) + 24 B per particle; "
+                        "the binding limit is the memory-side atomic request rate: one 16-byte request per sample, "
+                        "atomic.peak = live probe of that request shape"}
     # DE + tonemap of the finished histogram (HBM-bound pass), timed separately
     out = torch.empty((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
     de = rf.DEParams(wl.flame.estimator_radius, wl.flame.estimator_min, wl.flame.estimator_curve)

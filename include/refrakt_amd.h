@@ -128,7 +128,7 @@ typedef struct rf_sim_params {
   int32_t device;             /* HIP device ordinal */
   int32_t engine;             /* RF_ENGINE_* */
   int32_t hist_mode;          /* RF_HIST_* (fast engine) */
-  int32_t block_waves;        /* fast engine: waves per workgroup (4, 8 or 16); 0 -> 4 */
+  int32_t block_waves;        /* fast engine: waves per workgroup (4 or 8); 0 -> 4 */
 } rf_sim_params;
 
 /* Density estimation parameters (src/main.cpp:500-511). */

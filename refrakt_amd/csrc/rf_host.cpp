@@ -2,8 +2,8 @@
 //
 // The orchestration mirrors src/flame.cpp (set_sim_parameters, warmup,
 // draw_to_bins) and the inline DE/tonemap passes of src/main.cpp:490-535;
-// the device work is done by the kernels of rf_kernels.hip on one HIP stream
-// per context.
+// the device work is done by the kernels of rf_k_compat.hip, rf_k_fast.hip and
+// rf_k_misc.hip on one HIP stream per context.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -295,7 +295,7 @@ static int ensure_compat(rf_ctx* ctx) {
 
 static int fast_waves(const rf_ctx* ctx) {
   const int w = ctx->sp.block_waves;
-  return (w == 8 || w == 16) ? w : 4;
+  return w == 8 ? 8 : 4;
 }
 
 static int ensure_perms(rf_ctx* ctx) {
@@ -528,6 +528,21 @@ int rf_create(const rf_sim_params* p, rf_ctx** out) {
   rf_ctx* ctx = new rf_ctx();
   ctx->sp = *p;
   if (ctx->sp.shuffle_count == 0) ctx->sp.shuffle_count = 1024;
+  if (p->block_waves != 0 && p->block_waves != 4 && p->block_waves != 8) {
+    g_create_err = "block_waves must be 0 (default 4), 4 or 8";
+    delete ctx;
+    return RF_ERR_ARG;
+  }
+  if (p->engine != RF_ENGINE_COMPAT && p->engine != RF_ENGINE_FAST) {
+    g_create_err = "unknown engine";
+    delete ctx;
+    return RF_ERR_ARG;
+  }
+  if (p->hist_mode < 0 || p->hist_mode > 3) {
+    g_create_err = "unknown hist mode";
+    delete ctx;
+    return RF_ERR_ARG;
+  }
   int rc = validate_sim(ctx, p->total_particles, p->temporal_samples);
   if (rc) { g_create_err = ctx->err; delete ctx; return rc; }
   hipError_t e = hipSetDevice(p->device);

@@ -66,3 +66,19 @@ def test_context_creation_fails_loudly_without_gpu(rf):
         assert "rf_create" in str(e)
     else:
         raise AssertionError("rf_create succeeded without a GPU")
+
+
+def test_create_rejects_bad_sim_parameters(rf):
+    """Argument checks run before any HIP call, so they hold on the CPU too."""
+    from refrakt_amd import api
+    L = rf.lib()
+    for kw, word in [({"block_waves": 16}, "block_waves"), ({"engine": 7}, "engine"),
+                     ({"hist_mode": 9}, "hist mode")]:
+        args = dict(total_particles=1 << 16, temporal_samples=1, shuffle_count=0, seed=1, device=0,
+                    engine=api.RF_ENGINE_FAST, hist_mode=1, block_waves=4)
+        args.update(kw)
+        sp = api.SimParams(**args)
+        out = api.C.c_void_p()
+        assert L.rf_create(api.C.byref(sp), api.C.byref(out)) == -1  # RF_ERR_ARG
+        assert word in L.rf_last_error(None).decode()
+        assert not out.value
