@@ -33,8 +33,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="cfg2", choices=["cfg1", "cfg2", "cfg3"])
-    ap.add_argument("--hist-mode", type=int, default=1, help="1 quad, 2 wave-aggregated, 3 naive")
-    ap.add_argument("--waves", type=int, default=4)
+    ap.add_argument("--hist-mode", type=int, default=0, help="0 auto, 1 quad, 2 wave-aggregated, 3 naive, 4 LDS bin cache")
+    ap.add_argument("--waves", type=int, default=0, help="waves per workgroup: 4, 8, or 0 for the library default (8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-probe", action="store_true")
@@ -102,6 +102,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
+    mode_used, lds_hits = fh.hist_mode()
     value = drawn / wall
     attempted = wl.particles * wl.draw_passes * args.steps
 
@@ -123,14 +124,15 @@ def main():
     peak_atomic = probe * 4 / 1e9 if probe else None
     roofline = {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
-                "traffic": MEASURED_TRAFFIC.get((args.config, args.hist_mode, args.waves)),
+                "traffic": MEASURED_TRAFFIC.get((args.config, args.hist_mode, args.waves or 8)),
                 "kernel": "k_fast", "kernel_avg_ms": round(k_avg, 3),
                 "atomic": {"achieved": round(atomic_g, 2), "peak": round(peak_atomic, 2) if peak_atomic else None,
                            "unit": "G f32 atomic adds/s",
                            "frac": round(atomic_g / peak_atomic, 4) if peak_atomic else None},
                 "note": "algorithmic bytes = 32 B per binned sample (16 B RGBA read + write) + 24 B per particle; "
-                        "the binding limit is the memory-side atomic request rate: one 16-byte request per sample, "
-                        "atomic.peak = live probe of that request shape"}
+                        "the binding limit is the memory-side atomic request rate: one 16-byte request per sample "
+                        "that misses the LDS bin cache; atomic.achieved counts every binned sample's 4 adds, "
+                        "atomic.peak = live probe of the 16-byte request shape"}
     # DE + tonemap of the finished histogram (HBM-bound pass), timed separately
     out = torch.empty((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
     de = rf.DEParams(wl.flame.estimator_radius, wl.flame.estimator_min, wl.flame.estimator_curve)
@@ -159,7 +161,8 @@ def main():
                                f"{len(wl.flame.xforms)} xforms{' + final' if wl.flame.final_xform else ''}, "
                                f"{wl.draw_passes} passes/step",
                    "binned_samples": drawn, "attempted_samples": attempted, "hist_mode": args.hist_mode,
-                   "block_waves": args.waves, "parallelism": "single GPU"},
+                   "hist_mode_used": mode_used, "lds_hit_fraction": round(lds_hits, 4) if lds_hits >= 0 else None,
+                   "block_waves": args.waves or 8, "parallelism": "single GPU"},
         "roofline": roofline,
         "de_tonemap_ms": round(de_ms, 3),
     }

@@ -65,11 +65,13 @@ extern "C" {
 #define RF_ENGINE_COMPAT 0
 #define RF_ENGINE_FAST 1
 
-/* Histogram accumulation strategies of the fast engine. */
-#define RF_HIST_AUTO 0     /* pick from the sampled footprint */
+/* Histogram accumulation strategies of the fast engine (the compat engine
+ * uses QUAD for AUTO and LDS). */
+#define RF_HIST_AUTO 0     /* LDS first; QUAD from then on if under 2 % of the samples hit the LDS tables */
 #define RF_HIST_QUAD 1     /* 4 lanes per RGBA bin: one 16-byte atomic request per sample */
 #define RF_HIST_WAVEAGG 2  /* merge same-bin hits inside a wave before the atomics */
 #define RF_HIST_NAIVE 3    /* 4 separate float atomics per sample (baseline) */
+#define RF_HIST_LDS 4      /* per-workgroup LDS table of hot bins, flushed once per launch; misses go QUAD */
 
 typedef struct rf_ctx rf_ctx;
 typedef struct rf_flame rf_flame;
@@ -128,7 +130,7 @@ typedef struct rf_sim_params {
   int32_t device;             /* HIP device ordinal */
   int32_t engine;             /* RF_ENGINE_* */
   int32_t hist_mode;          /* RF_HIST_* (fast engine) */
-  int32_t block_waves;        /* fast engine: waves per workgroup (4 or 8); 0 -> 4 */
+  int32_t block_waves;        /* fast engine: waves per workgroup (4 or 8); 0 -> 8 */
 } rf_sim_params;
 
 /* Density estimation parameters (src/main.cpp:500-511). */
@@ -254,6 +256,10 @@ int rf_shuffle_download(rf_ctx* ctx, uint32_t* host);
 int rf_sample_points_download(rf_ctx* ctx, float* host);
 /* Fast engine: the 16 LDS permutation rows (16 x 64*block_waves u16). */
 int rf_fast_perms_download(rf_ctx* ctx, uint16_t* host);
+/* Histogram strategy the flame's next fast-engine draw uses (resolves
+ * RF_HIST_AUTO) and, if a draw has used RF_HIST_LDS, the fraction of its binned
+ * samples whose bin was already in their workgroup's LDS table (else -1). */
+int rf_flame_hist_mode(const rf_flame* flame, double* lds_hit_fraction);
 /* Screen-space affine used by draw_to_bins (src/flame.cpp:289-296). */
 int rf_screen_affine(const rf_flame_desc* desc, uint32_t width, uint32_t height, float out[6]);
 

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "librefrakt_amd.so")
 RF_OK = 0
 RF_ENGINE_COMPAT = 0
 RF_ENGINE_FAST = 1
-RF_HIST_AUTO, RF_HIST_QUAD, RF_HIST_WAVEAGG, RF_HIST_NAIVE = 0, 1, 2, 3
+RF_HIST_AUTO, RF_HIST_QUAD, RF_HIST_WAVEAGG, RF_HIST_NAIVE, RF_HIST_LDS = 0, 1, 2, 3, 4
 MAX_XFORMS = 32
 MAX_VARS = 16
 MAX_PARAMS = 8
@@ -118,6 +118,7 @@ _SIGS = {
     "rf_rng_download": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "rf_shuffle_download": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "rf_sample_points_download": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "rf_flame_hist_mode": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "rf_screen_affine": (C.c_int, [C.POINTER(FlameDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     "rf_variation_count": (C.c_int, []),
     "rf_variation_name": (C.c_char_p, [C.c_int]),
@@ -335,8 +336,8 @@ class Context:
     """Simulation state of flame::set_sim_parameters plus one HIP stream."""
 
     def __init__(self, total_particles: int, temporal_samples: int = 512, shuffle_count: int = 1024,
-                 seed: int = 1, device: int = 0, engine: int = RF_ENGINE_FAST, hist_mode: int = RF_HIST_QUAD,
-                 block_waves: int = 4):
+                 seed: int = 1, device: int = 0, engine: int = RF_ENGINE_FAST, hist_mode: int = RF_HIST_AUTO,
+                 block_waves: int = 0):
         self.L = lib()
         sp = SimParams(total_particles, temporal_samples, shuffle_count, seed, device, engine, hist_mode, block_waves)
         h = C.c_void_p()
@@ -490,6 +491,14 @@ class FlameHandle:
         out = np.zeros((self.ctx.total_particles, 4), np.float32)
         self.ctx.check(self.ctx.L.rf_particles_download(self.ctx.h, self.h, _fptr(out)))
         return out
+
+    def hist_mode(self):
+        """(strategy of the next draw, LDS hit fraction of the last LDS draw or -1)."""
+        frac = C.c_double(-1.0)
+        m = self.ctx.L.rf_flame_hist_mode(self.h, C.byref(frac))
+        if m < 0:
+            self.ctx.check(m)
+        return m, frac.value
 
     def param_block(self) -> np.ndarray:
         return param_block(self.flame)

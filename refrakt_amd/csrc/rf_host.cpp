@@ -294,8 +294,10 @@ static int ensure_compat(rf_ctx* ctx) {
 }
 
 static int fast_waves(const rf_ctx* ctx) {
+  // 8 waves suit the LDS bin cache (its 2048-slot table shares the CU's LDS
+  // between two workgroups) and cost the direct modes about 1 %.
   const int w = ctx->sp.block_waves;
-  return w == 8 ? 8 : 4;
+  return w == 4 ? 4 : 8;
 }
 
 static int ensure_perms(rf_ctx* ctx) {
@@ -345,6 +347,7 @@ static int flame_upload(rf_flame* f, float tss_width) {
   HIPCHK(ctx, hipMemcpyAsync(f->palette.p, f->desc.palette, 256 * 16, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   f->needs_update = false;
+  f->auto_mode = RF_HIST_LDS;
   return RF_OK;
 }
 
@@ -426,6 +429,13 @@ static int do_warmup(rf_ctx* ctx, rf_flame* f, uint32_t num_passes, float tss, c
   return RF_OK;
 }
 
+// RF_HIST_AUTO keeps the LDS bin cache while at least this fraction of the
+// binned samples finds its bin in a workgroup's table.  Measured on MI355X the
+// cache already breaks even with the quad path at a 4 % hit fraction (a
+// 7680x4320 view of an 8-xform flame), so it is only dropped when the table
+// is nearly useless.
+static constexpr double kAutoLdsMinHits = 0.02;
+
 static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, const uint32_t* rows_in,
                    uint64_t* drawn) {
   if (!ctx || !f || !h || f->ctx != ctx || h->ctx != ctx) return set_err(ctx, RF_ERR_ARG, "bad context, flame or histogram");
@@ -434,7 +444,8 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
   const uint64_t P = ctx->sp.total_particles;
   float ss[6];
   screen_affine(f->desc, h->w, h->h, ss);
-  HIPCHK(ctx, hipMemsetAsync(ctx->counter.p, 0, 8, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(ctx->counter.p, 0, 16, ctx->stream));
+  const int hm = ctx->sp.hist_mode == RF_HIST_AUTO ? f->auto_mode : ctx->sp.hist_mode;
   if (!ctx->ev[2]) { HIPCHK(ctx, hipEventCreate(&ctx->ev[2])); HIPCHK(ctx, hipEventCreate(&ctx->ev[3])); }
   if (ctx->sp.engine == RF_ENGINE_COMPAT) {
     int rc = ensure_compat(ctx);
@@ -468,16 +479,20 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     a.W = h->w; a.H = h->h;
     memcpy(a.ss, ss, sizeof(ss));
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    if (num_iter) HIPCHK(ctx, rfk::launch_fast(a, P, fast_waves(ctx), ctx->sp.hist_mode, ctx->stream));
+    if (num_iter) HIPCHK(ctx, rfk::launch_fast(a, P, fast_waves(ctx), hm, ctx->stream));
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-  uint64_t cnt = 0;
-  HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->counter.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+  uint64_t cnt[2] = {0, 0};
+  HIPCHK(ctx, hipMemcpyAsync(cnt, ctx->counter.p, 16, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->sp.engine == RF_ENGINE_FAST && hm == RF_HIST_LDS && cnt[0]) {
+    f->last_lds_hit_fraction = (double)cnt[1] / (double)cnt[0];
+    if (ctx->sp.hist_mode == RF_HIST_AUTO && f->last_lds_hit_fraction < kAutoLdsMinHits) f->auto_mode = RF_HIST_QUAD;
+  }
   float ms = 0.0f;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
   ctx->last_draw_ms = ms;
-  if (drawn) *drawn = cnt;
+  if (drawn) *drawn = cnt[0];
   return RF_OK;
 }
 
@@ -538,7 +553,7 @@ int rf_create(const rf_sim_params* p, rf_ctx** out) {
     delete ctx;
     return RF_ERR_ARG;
   }
-  if (p->hist_mode < 0 || p->hist_mode > 3) {
+  if (p->hist_mode < RF_HIST_AUTO || p->hist_mode > RF_HIST_LDS) {
     g_create_err = "unknown hist mode";
     delete ctx;
     return RF_ERR_ARG;
@@ -552,7 +567,7 @@ int rf_create(const rf_sim_params* p, rf_ctx** out) {
   // kernels read or write them, and the caller's later work sees our results.
   e = hipStreamCreate(&ctx->stream);
   if (e != hipSuccess) { g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e); delete ctx; return RF_ERR_HIP; }
-  e = ctx->counter.alloc(8);
+  e = ctx->counter.alloc(16);
   if (e != hipSuccess) { g_create_err = std::string("hipMalloc: ") + hipGetErrorString(e); rf_destroy(ctx); return RF_ERR_HIP; }
   ctx->ppt = p->total_particles / p->temporal_samples;
   ctx->shuffle_count = ctx->sp.shuffle_count;
@@ -587,7 +602,7 @@ int rf_set_sim_parameters(rf_ctx* ctx, uint64_t P, uint32_t T, uint32_t R) {
 int rf_set_engine(rf_ctx* ctx, int32_t engine, int32_t hist_mode) {
   if (!ctx) return RF_ERR_ARG;
   if (engine != RF_ENGINE_COMPAT && engine != RF_ENGINE_FAST) return set_err(ctx, RF_ERR_ARG, "unknown engine");
-  if (hist_mode < 0 || hist_mode > 3) return set_err(ctx, RF_ERR_ARG, "unknown hist mode");
+  if (hist_mode < RF_HIST_AUTO || hist_mode > RF_HIST_LDS) return set_err(ctx, RF_ERR_ARG, "unknown hist mode");
   if (engine != ctx->sp.engine) ctx->generation++;  // particle layouts differ between engines
   ctx->sp.engine = engine;
   ctx->sp.hist_mode = hist_mode;
@@ -668,6 +683,13 @@ int rf_flame_validate(const rf_flame_desc* d, char* err, size_t err_len) {
   delete b;
   if (!ok && err && err_len) snprintf(err, err_len, "%s", e.c_str());
   return ok ? RF_OK : RF_ERR_ARG;
+}
+
+int rf_flame_hist_mode(const rf_flame* f, double* frac) {
+  if (!f) return RF_ERR_ARG;
+  if (frac) *frac = f->last_lds_hit_fraction;
+  const int m = f->ctx->sp.hist_mode;
+  return m == RF_HIST_AUTO ? f->auto_mode : m;
 }
 
 int rf_screen_affine(const rf_flame_desc* d, uint32_t w, uint32_t h, float out[6]) {

@@ -63,7 +63,7 @@ struct rf_ctx {
   // fast engine
   rfh::DevBuf perms;
   int perm_waves = 0;
-  rfh::DevBuf counter;         // u64 drawn counter
+  rfh::DevBuf counter;         // u64 [drawn, samples added in the LDS bin cache]
   rfh::DevBuf probe;           // atomic probe scratch
   rf_hist* render_hist = nullptr;
   rfh::DevBuf render_img;
@@ -83,6 +83,12 @@ struct rf_flame {
   rfh::DevBuf palette;         // 256 float4
   rfh::DevBuf local;           // particles
   bool have_particles = false;
+  // RF_HIST_AUTO: the strategy the next draw uses.  It starts with the LDS
+  // bin cache and falls back to the quad path for the rest of the flame's
+  // life (until its next upload) if the first draw added fewer than
+  // kAutoLdsMinHits of its samples in LDS.
+  int auto_mode = RF_HIST_LDS;
+  double last_lds_hit_fraction = -1.0;
 };
 
 struct rf_hist {

@@ -1,5 +1,5 @@
 // Internal interface between the host library (rf_host.cpp) and the kernel
-// launchers (rf_kernels.hip).
+// launchers (rf_k_compat.hip, rf_k_fast.hip, rf_k_misc.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -38,6 +38,7 @@ struct FastArgs {
   uint32_t launch;               // launch counter (RNG streams)
   uint32_t ppt;
   uint32_t n_iter;
+  uint32_t n_batches;            // batches of 64*waves particles (set by launch_fast)
   uint32_t first_run, do_draw;
   uint32_t W, H;
   float ss[6];

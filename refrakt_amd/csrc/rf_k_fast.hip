@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "rf_kern_common.hpp"
 
 using namespace rfd;
@@ -11,14 +13,21 @@ namespace {
 using namespace detail;
 
 // ------------------------------------------------------------ fast engine
-// Persistent chaos game: a workgroup owns 64*WAVES particles for all n_iter
-// iterations.  Per iteration each wave draws one xform from its own
-// wave-uniform stream (so the variation switch never diverges), applies it,
-// bins, and then all particles of the workgroup are permuted through LDS with
-// one of NPERM random permutations chosen by a workgroup-uniform stream,
+// Persistent chaos game.  A batch is 64*WAVES particles; a workgroup takes
+// batches blockIdx.x, blockIdx.x + gridDim.x, ... and runs each for all
+// n_iter iterations with the particles in registers.  Per iteration each wave
+// draws one xform from its own wave-uniform stream (so the variation switch
+// never diverges), applies it, bins, and then the batch is permuted through
+// LDS with one of NPERM random permutations chosen by a batch-uniform stream,
 // which decorrelates the xform sequences of particles sharing a wave (the
-// role of the reference's shuffle rows, src/shuffle_buffers.cpp).
-template <int WAVES, int HM>
+// role of the reference's shuffle rows, src/shuffle_buffers.cpp).  All random
+// streams are keyed by particle, wave-of-batch and batch index, never by the
+// workgroup, so the results do not depend on the grid size.
+//
+// HM 1/2/3 bin straight to the histogram and launch one workgroup per batch;
+// HM 4 (LDS bin cache) launches one workgroup per resident slot so that each
+// workgroup's LDS table spans many batches before it is flushed.
+template <int WAVES, int HM, int LOGK>
 __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __restrict__ blocks) {
   constexpr int N = 64 * WAVES;
   // Double-buffered exchange: iteration `it` writes buffer it&1, so one
@@ -29,96 +38,138 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __
   // The permutation rows live in LDS: a global load inside the loop would sit
   // behind the iteration's atomics in the in-order vmcnt queue and stall.
   __shared__ uint16_t s_perm[NPERM * N];
+  __shared__ LdsBins<LOGK> s_bins;
   const uint32_t tid = threadIdx.x;
-  const uint32_t base = blockIdx.x * (uint32_t)N;
-  const uint32_t pid = base + tid;
-  // Integer division runs on the VALU, so the quotient is not known to be
-  // uniform; without readfirstlane every Block access would be a vector load.
-  const uint32_t t = __builtin_amdgcn_readfirstlane(base / g.ppt);
-  CBlock* blk = as_constant(blocks + (size_t)t * g.block_stride);
   for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palette[i];
   for (uint32_t i = tid; i < (uint32_t)(NPERM * N); i += N) s_perm[i] = g.perms[i];
-
-  Jsf32 lrng = jsf_from_philox(g.seed, pid, g.launch, 0u);
-  const uint32_t wid = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + (tid >> 6));
-  Jsf32 wrng = jsf_from_philox(g.seed, wid, g.launch, 1u);
-  Jsf32 grng = jsf_from_philox(g.seed, blockIdx.x, g.launch, 2u);
-
-  float x, y, c;
-  if (g.first_run) {
-    // Hammersley point (src/hammersley.cpp:29-48) of index pid mod ppt, jittered
-    // like the first run of shaders/flame.glsl:63-66.
-    const uint32_t cnt = g.ppt;
-    const uint32_t mx = (cnt % 2u == 0u) ? cnt : (1u << (32 - __clz(cnt - 1u)));
-    const float inv_max = rf_div_rn(1.0f, (float)mx);
-    const uint32_t rbits = 31u - __clz(mx);
-    const uint32_t i = pid % cnt;
-    const uint32_t fl = __brev(i) >> (32u - rbits);
-    x = ((float)i * inv_max) * 2.0f - 1.0f;
-    y = ((float)fl * inv_max) * 2.0f - 1.0f;
-    c = 0.0f;
-    const float r1 = lrng.randf();
-    const float r2 = lrng.randf();
-    const float s = ((r1 * 0.1f) * PI) * 2.0f;
-    float sn, cs;
-    rf_sincos01(rf_sqrt_rn(r2), &sn, &cs);
-    x = x + s * sn;
-    y = y + s * cs;
-  } else {
-    const float4 p = g.pos[pid];
-    x = p.x; y = p.y; c = p.z;
-  }
+  if constexpr (HM == 4) lds_bins_clear(s_bins);
   __syncthreads();
 
   unsigned long long drawn = 0;
-  for (uint32_t it = 0; it < g.n_iter; ++it) {
-    // The draw is wave-uniform; selecting on the raw integer keeps xid in SGPRs
-    // (float compares would run on the VALU, make xid look divergent and turn
-    // every xform/term access into vector loads behind the atomics).
-    const int xid = select_xform_raw(blk, wrng.next_a());
-    XOut o = apply_xform(blk, xid, x, y, c, g.first_run && it == 0, lrng);
-    if (rf_badval(o.x) || rf_badval(o.y)) {
-      // A particle that left the finite plane restarts at a random point and
-      // is not drawn this iteration (the reference leaves it NaN forever,
-      // shaders/flame.glsl:70 is commented out).
-      o.x = lrng.randf() * 2.0f - 1.0f;
-      o.y = lrng.randf() * 2.0f - 1.0f;
-      o.w = 0.0f;
+  uint32_t hits = 0;
+  for (uint32_t batch = blockIdx.x; batch < g.n_batches; batch += gridDim.x) {
+    const uint32_t base = batch * (uint32_t)N;
+    const uint32_t pid = base + tid;
+    // Integer division runs on the VALU, so the quotient is not known to be
+    // uniform; without readfirstlane every Block access would be a vector load.
+    const uint32_t t = __builtin_amdgcn_readfirstlane(base / g.ppt);
+    CBlock* blk = as_constant(blocks + (size_t)t * g.block_stride);
+
+    Jsf32 lrng = jsf_from_philox(g.seed, pid, g.launch, 0u);
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(batch * WAVES + (tid >> 6));
+    Jsf32 wrng = jsf_from_philox(g.seed, wid, g.launch, 1u);
+    Jsf32 grng = jsf_from_philox(g.seed, batch, g.launch, 2u);
+
+    float x, y, c;
+    if (g.first_run) {
+      // Hammersley point (src/hammersley.cpp:29-48) of index pid mod ppt, jittered
+      // like the first run of shaders/flame.glsl:63-66.
+      const uint32_t cnt = g.ppt;
+      const uint32_t mx = (cnt % 2u == 0u) ? cnt : (1u << (32 - __clz(cnt - 1u)));
+      const float inv_max = rf_div_rn(1.0f, (float)mx);
+      const uint32_t rbits = 31u - __clz(mx);
+      const uint32_t i = pid % cnt;
+      const uint32_t fl = __brev(i) >> (32u - rbits);
+      x = ((float)i * inv_max) * 2.0f - 1.0f;
+      y = ((float)fl * inv_max) * 2.0f - 1.0f;
+      c = 0.0f;
+      const float r1 = lrng.randf();
+      const float r2 = lrng.randf();
+      const float s = ((r1 * 0.1f) * PI) * 2.0f;
+      float sn, cs;
+      rf_sincos01(rf_sqrt_rn(r2), &sn, &cs);
+      x = x + s * sn;
+      y = y + s * cs;
+    } else {
+      const float4 p = g.pos[pid];
+      x = p.x; y = p.y; c = p.z;
     }
-    x = o.x; y = o.y; c = o.c;
-    if (g.do_draw) {
-      const bool v = draw_point<HM>(blk, o, lrng, g.ss, g.W, g.H, g.bins, s_pal);
-      drawn += (unsigned long long)__popcll(__ballot(v));
+
+    for (uint32_t it = 0; it < g.n_iter; ++it) {
+      // The draw is wave-uniform; selecting on the raw integer keeps xid in SGPRs
+      // (float compares would run on the VALU, make xid look divergent and turn
+      // every xform/term access into vector loads behind the atomics).
+      const int xid = select_xform_raw(blk, wrng.next_a());
+      XOut o = apply_xform(blk, xid, x, y, c, g.first_run && it == 0, lrng);
+      if (rf_badval(o.x) || rf_badval(o.y)) {
+        // A particle that left the finite plane restarts at a random point and
+        // is not drawn this iteration (the reference leaves it NaN forever,
+        // shaders/flame.glsl:70 is commented out).
+        o.x = lrng.randf() * 2.0f - 1.0f;
+        o.y = lrng.randf() * 2.0f - 1.0f;
+        o.w = 0.0f;
+      }
+      x = o.x; y = o.y; c = o.c;
+      if (g.do_draw) {
+        const Located L = locate(blk, o, lrng, g.ss, g.W, g.H);
+        if constexpr (HM == 4) bin_lds(s_bins, g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal, hits);
+        else bin_sample<HM>(g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal);
+        drawn += (unsigned long long)__popcll(__ballot(L.valid));
+      }
+      const uint32_t row = grng.next_a() & (uint32_t)(NPERM - 1);
+      const uint32_t dst = s_perm[row * N + tid];
+      const uint32_t b = it & 1u;
+      s_x[b][dst] = x; s_y[b][dst] = y; s_c[b][dst] = c;
+      lds_barrier();
+      x = s_x[b][tid]; y = s_y[b][tid]; c = s_c[b][tid];
     }
-    const uint32_t row = grng.next_a() & (uint32_t)(NPERM - 1);
-    const uint32_t dst = s_perm[row * N + tid];
-    const uint32_t b = it & 1u;
-    s_x[b][dst] = x; s_y[b][dst] = y; s_c[b][dst] = c;
+    g.pos[pid] = make_float4(x, y, c, 0.0f);
+    // The next batch's first exchange writes buffer 0, which slower waves may
+    // still be reading from this batch's last iteration.
     lds_barrier();
-    x = s_x[b][tid]; y = s_y[b][tid]; c = s_c[b][tid];
   }
-  g.pos[pid] = make_float4(x, y, c, 0.0f);
-  if (g.do_draw) count_drawn(g.counter, drawn);
+  if (g.do_draw) {
+    count_drawn(g.counter, drawn);
+    if constexpr (HM == 4) {
+      __syncthreads();
+      flush_lds(s_bins, g.bins);
+      const unsigned long long wh = (unsigned long long)wave_sum_u32(hits);
+      if ((threadIdx.x & (WAVE - 1)) == 0 && wh) atomicAdd(g.counter + 1, wh);
+    }
+  }
 }
 
 }  // namespace
 
+template <int W, int HM, int LOGK>
+static hipError_t launch_fast_k(const FastArgs& a0, uint64_t n, hipStream_t s) {
+  FastArgs a = a0;
+  a.n_batches = (uint32_t)(n / (64u * W));
+  uint32_t grid = a.n_batches;
+  if (HM == 4) {
+    // one workgroup per resident slot (cached per instantiation)
+    static int resident = 0;
+    if (!resident) {
+      int dev = 0, cus = 0, per_cu = 0;
+      hipError_t e = hipGetDevice(&dev);
+      if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_fast<W, HM, LOGK>),
+                                                         64 * W, 0);
+      if (e != hipSuccess) return e;
+      resident = cus * (per_cu > 0 ? per_cu : 1);
+    }
+    grid = std::min<uint32_t>(grid, (uint32_t)resident);
+  }
+  hipLaunchKernelGGL((k_fast<W, HM, LOGK>), dim3(grid), dim3(64 * W), 0, s, a, a.blocks);
+  return hipGetLastError();
+}
+
 template <int W>
-static void launch_fast_w(const FastArgs& a, uint64_t n, int hm, hipStream_t s) {
-  dim3 grid((uint32_t)(n / (64u * W))), block(64 * W);
+static hipError_t launch_fast_w(const FastArgs& a, uint64_t n, int hm, hipStream_t s) {
   switch (hm) {
-    case 2: hipLaunchKernelGGL((k_fast<W, 2>), grid, block, 0, s, a, a.blocks); break;
-    case 3: hipLaunchKernelGGL((k_fast<W, 3>), grid, block, 0, s, a, a.blocks); break;
-    default: hipLaunchKernelGGL((k_fast<W, 1>), grid, block, 0, s, a, a.blocks); break;
+    case 2: return launch_fast_k<W, 2, 0>(a, n, s);
+    case 3: return launch_fast_k<W, 3, 0>(a, n, s);
+    case 4: return launch_fast_k<W, 4, (W == 8 ? 11 : 10)>(a, n, s);
+    default: return launch_fast_k<W, 1, 0>(a, n, s);
   }
 }
 
 hipError_t launch_fast(const FastArgs& a, uint64_t n_particles, int waves, int hist_mode, hipStream_t s) {
   switch (waves) {
-    case 8: launch_fast_w<8>(a, n_particles, hist_mode, s); break;
-    default: launch_fast_w<4>(a, n_particles, hist_mode, s); break;
+    case 8: return launch_fast_w<8>(a, n_particles, hist_mode, s);
+    default: return launch_fast_w<4>(a, n_particles, hist_mode, s);
   }
-  return hipGetLastError();
 }
 
 }  // namespace rfk

@@ -59,6 +59,12 @@ __device__ __forceinline__ void bin_naive(float* __restrict__ bins, uint32_t bid
   atomic_add_f32(p + 3, w);
 }
 
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+  return v;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
@@ -98,6 +104,101 @@ __device__ __forceinline__ void bin_waveagg(float* __restrict__ bins, uint32_t b
   bin_quad(bins, bidx, (rem >> lane) & 1ull, pidx, w, pal);
 }
 
+// Adds one RGBA value per lane (`v`) to bin `bidx` with the quad layout of
+// bin_quad: 4 lanes per bin, one 16-byte request per value.
+__device__ __forceinline__ void bin_quad_vals(float* __restrict__ bins, uint32_t bidx, bool valid, float4 v) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint64_t vm = __ballot(valid);
+  if (vm == 0) return;
+  const int ch = lane & 3;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t grp = (uint32_t)(vm >> (16 * k)) & 0xffffu;
+    if (grp == 0) continue;
+    const int src = 16 * k + (lane >> 2);
+    const uint32_t b = (uint32_t)__shfl((int)bidx, src);
+    const float vx = __shfl(v.x, src), vy = __shfl(v.y, src), vz = __shfl(v.z, src), vw = __shfl(v.w, src);
+    if ((grp >> (lane >> 2)) & 1u) {
+      const float val = ch == 0 ? vx : ch == 1 ? vy : ch == 2 ? vz : vw;
+      atomic_add_f32(bins + (size_t)b * 4 + ch, val);
+    }
+  }
+}
+
+// HIST_LDS: each workgroup keeps an open-addressed table of 2^LOGK bins in
+// LDS (key = bin + 1, 0 = free; claimed on first touch, never evicted) and
+// adds the samples that find or claim a slot there with LDS atomics; the
+// rest take the quad path.  The table is added to the histogram once, when
+// the workgroup finishes (flush_lds), so a bin hit thousands of times by one
+// workgroup costs one global request instead of thousands.  This is what a
+// concentrated footprint needs: same-address atomics serialise at the
+// memory-side atomic unit (a single hot bin sustains under 0.1 G adds/s).
+constexpr int LDS_PROBES = 4;
+
+template <int LOGK>
+struct LdsBins {
+  uint32_t key[1u << LOGK];
+  float4 val[1u << LOGK];
+};
+template <>
+struct LdsBins<0> {};  // modes without the cache: no LDS
+
+template <int LOGK>
+__device__ __forceinline__ void lds_bins_clear(LdsBins<LOGK>& t) {
+  for (uint32_t i = threadIdx.x; i < (1u << LOGK); i += blockDim.x) {
+    t.key[i] = 0u;
+    t.val[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// `hits` counts the samples whose bin was already in the table (the claim
+// of a free slot is not counted): the measure of what the table saves.
+template <int LOGK>
+__device__ __forceinline__ void bin_lds(LdsBins<LOGK>& t, float* __restrict__ bins, uint32_t bidx, bool valid,
+                                        uint32_t pidx, float w, const float4* __restrict__ pal, uint32_t& hits) {
+  constexpr uint32_t K = 1u << LOGK;
+  int slot = -1;
+  bool claimed = false;
+  if (valid) {
+    const uint32_t key = bidx + 1u;
+    uint32_t s = (bidx * 0x9E3779B1u) >> (32 - LOGK);
+    for (int p = 0; p < LDS_PROBES; ++p) {
+      uint32_t k = __hip_atomic_load(&t.key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (k == 0u) {
+        uint32_t expected = 0u;
+        claimed = __hip_atomic_compare_exchange_strong(&t.key[s], &expected, key, __ATOMIC_RELAXED,
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        k = claimed ? key : expected;
+      }
+      if (k == key) { slot = (int)s; break; }
+      s = (s + 1u) & (K - 1u);
+    }
+  }
+  if (slot >= 0) {
+    const float4 c = pal[pidx];
+    float* v = reinterpret_cast<float*>(&t.val[slot]);
+    __hip_atomic_fetch_add(v + 0, c.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(v + 1, c.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(v + 2, c.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(v + 3, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // Only re-hits count: a table that is merely being filled saves nothing.
+    if (!claimed) ++hits;
+  }
+  bin_quad(bins, bidx, valid && slot < 0, pidx, w, pal);
+}
+
+// Adds every claimed slot to the histogram.  Call after a full barrier; the
+// loop bound is workgroup-uniform so every wave takes part in every ballot.
+template <int LOGK>
+__device__ __forceinline__ void flush_lds(LdsBins<LOGK>& t, float* __restrict__ bins) {
+  for (uint32_t s0 = 0; s0 < (1u << LOGK); s0 += blockDim.x) {
+    const uint32_t s = s0 + threadIdx.x;
+    const uint32_t k = s < (1u << LOGK) ? t.key[s] : 0u;
+    const float4 v = k ? t.val[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    bin_quad_vals(bins, k - 1u, k != 0u, v);
+  }
+}
+
 template <int HM>
 __device__ __forceinline__ void bin_sample(float* __restrict__ bins, uint32_t bidx, bool valid,
                                            uint32_t pidx, float w, const float4* __restrict__ pal) {
@@ -106,14 +207,20 @@ __device__ __forceinline__ void bin_sample(float* __restrict__ bins, uint32_t bi
   else bin_quad(bins, bidx, valid, pidx, w, pal);
 }
 
-// Final xform, screen affine, floor, bounds and opacity test, then the bin
-// add (shaders/flame.glsl:74-87).  Float comparisons replace the integer
-// ones of the reference: floor(s) >= 0 <=> s >= 0 and floor(s) < W <=> s < W,
-// and NaN fails both (the reference's int conversion of NaN is undefined).
-template <int HM, class Rng, class BP>
-__device__ __forceinline__ bool draw_point(BP blk, const XOut& o, Rng& rng,
-                                           const float ss[6], uint32_t W, uint32_t H,
-                                           float* __restrict__ bins, const float4* __restrict__ pal) {
+// Final xform, screen affine, floor, bounds and opacity test
+// (shaders/flame.glsl:74-87).  Float comparisons replace the integer ones of
+// the reference: floor(s) >= 0 <=> s >= 0 and floor(s) < W <=> s < W, and NaN
+// fails both (the reference's int conversion of NaN is undefined).
+struct Located {
+  uint32_t bidx;   // (H - 1 - floor(sy)) * W + floor(sx); 0 when !valid
+  uint32_t pidx;   // palette index
+  float w;         // opacity weight
+  bool valid;
+};
+
+template <class Rng, class BP>
+__device__ __forceinline__ Located locate(BP blk, const XOut& o, Rng& rng, const float ss[6], uint32_t W,
+                                          uint32_t H) {
   float rx = o.x, ry = o.y, rc = o.c, rw = o.w;
   if (blk->has_final) {
     const XOut f = apply_xform(blk, FINAL, o.x, o.y, o.c, false, rng);
@@ -121,14 +228,25 @@ __device__ __forceinline__ bool draw_point(BP blk, const XOut& o, Rng& rng,
   }
   const float sx = fmaf(ss[0], rx, fmaf(ss[2], ry, ss[4]));
   const float sy = fmaf(ss[1], rx, fmaf(ss[3], ry, ss[5]));
-  const bool valid = (sx >= 0.0f) && (sx < (float)W) && (sy >= 0.0f) && (sy < (float)H) && (rw > 0.0f);
-  uint32_t bidx = 0;
-  if (valid) {
+  Located L;
+  L.valid = (sx >= 0.0f) && (sx < (float)W) && (sy >= 0.0f) && (sy < (float)H) && (rw > 0.0f);
+  L.bidx = 0;
+  if (L.valid) {
     const uint32_t cx = (uint32_t)floorf(sx), cy = (uint32_t)floorf(sy);
-    bidx = (H - cy - 1u) * W + cx;
+    L.bidx = (H - cy - 1u) * W + cx;
   }
-  bin_sample<HM>(bins, bidx, valid, rf_pal_index(rc), rw, pal);
-  return valid;
+  L.pidx = rf_pal_index(rc);
+  L.w = rw;
+  return L;
+}
+
+template <int HM, class Rng, class BP>
+__device__ __forceinline__ bool draw_point(BP blk, const XOut& o, Rng& rng,
+                                           const float ss[6], uint32_t W, uint32_t H,
+                                           float* __restrict__ bins, const float4* __restrict__ pal) {
+  const Located L = locate(blk, o, rng, ss, W, H);
+  bin_sample<HM>(bins, L.bidx, L.valid, L.pidx, L.w, pal);
+  return L.valid;
 }
 
 // Workgroup barrier that orders LDS only.  __syncthreads() also waits for

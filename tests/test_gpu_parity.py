@@ -94,7 +94,7 @@ def test_compat_bit_exact_vs_oracle(rf, orc, gpu_available):
 
 
 # -------------------------------------------------------------- fast engine
-@pytest.mark.parametrize("hist_mode", [1, 2, 3])
+@pytest.mark.parametrize("hist_mode", [1, 2, 3, 4])
 def test_fast_bit_exact_vs_oracle_replica(rf, orc, gpu_available, hist_mode):
     P, T, W, H, seed = 1 << 13, 1, 192, 160, 99
     fl = _linear_flame(rf, 21, W=W, H=H)
@@ -261,7 +261,7 @@ def test_fused_density_tonemap_matches_split(rf, gpu_available):
 
 
 # -------------------------------------------------------- conservation, size
-@pytest.mark.parametrize("hist_mode", [1, 2])
+@pytest.mark.parametrize("hist_mode", [1, 2, 4, 0])
 def test_contention_counts_conserved(rf, gpu_available, hist_mode):
     """A strongly contractive flame: every binned sample adds exactly 1 to alpha."""
     from refrakt_amd import configs
@@ -278,6 +278,58 @@ def test_contention_counts_conserved(rf, gpu_available, hist_mode):
     assert g[..., 3].max() < 2 ** 24
     assert drawn == (1 << 18) * 32
     assert int(g[..., 3].astype(np.float64).sum()) == drawn
+    if hist_mode in (0, 4):
+        mode, frac = fh.hist_mode()
+        assert mode == 4 and frac > 0.95  # a few dozen hot bins: nearly every sample re-hits its LDS slot
+
+
+@pytest.mark.parametrize("waves", [4, 8])
+def test_lds_cache_matches_quad_over_many_batches(rf, gpu_available, waves):
+    """2^20 particles: each workgroup of the LDS mode runs several batches and
+    flushes once; trajectories and alpha counts equal the quad mode's exactly."""
+    from refrakt_amd import configs
+    wl = configs.cfg1()
+    out = {}
+    for mode in (1, 4):
+        ctx = _ctx(rf, 1 << 20, 1, rf.RF_ENGINE_FAST, hist_mode=mode, seed=7, waves=waves)
+        fh = ctx.flame(wl.flame)
+        hist = ctx.histogram(wl.width, wl.height)
+        fh.warmup(19)
+        drawn = fh.draw_to_bins(hist, 8)
+        out[mode] = (drawn, hist.download(), fh.particles(), fh.hist_mode())
+    d1, h1, p1, _ = out[1]
+    d4, h4, p4, (m4, frac) = out[4]
+    assert d1 == d4 > 0
+    assert np.array_equal(p1.view(np.uint32), p4.view(np.uint32))
+    assert np.array_equal(h1[..., 3], h4[..., 3])
+    # float32 sums of up to ~3e4 terms in different orders: up to n * 2^-24 apart
+    np.testing.assert_allclose(h4[..., :3], h1[..., :3], rtol=2e-3, atol=1e-3)
+    assert m4 == 4 and 0.2 < frac <= 1.0
+
+
+def test_auto_mode_falls_back_on_a_spread_footprint(rf, gpu_available):
+    """Four half-scale maps tile the square, so hits spread evenly over all
+    2048^2 bins and the LDS tables catch almost none: AUTO drops them after one
+    draw and the histogram stays exact across the switch."""
+    W = H = 2048
+    xs = [rf.Xform(affine=[0.5, 0.0, 0.0, 0.5, ex, fy], variations={"linear": 1.0}, weight=1.0, color=0.25 * k,
+                   color_speed=0.5, opacity=1.0)
+          for k, (ex, fy) in enumerate([(-0.5, -0.5), (0.5, -0.5), (-0.5, 0.5), (0.5, 0.5)])]
+    fl = rf.Flame(xforms=xs, palette=np.ones((256, 4), np.float32), size=[W, H], center=[0.0, 0.0],
+                  scale=W / 2.0, rotate=0.0)
+    ctx = _ctx(rf, 1 << 20, 1, rf.RF_ENGINE_FAST, hist_mode=0, waves=8)
+    fh = ctx.flame(fl)
+    hist = ctx.histogram(W, H)
+    fh.warmup(19)
+    assert fh.hist_mode()[0] == 4
+    d = fh.draw_to_bins(hist, 4)
+    mode, frac = fh.hist_mode()
+    assert 0.0 <= frac < 0.02 and mode == 1
+    d += fh.draw_to_bins(hist, 4)
+    assert fh.hist_mode()[0] == 1
+    g = hist.download()[..., 3]
+    assert int(g.astype(np.float64).sum()) == d == (1 << 20) * 8
+    assert (g > 0).mean() > 0.8  # 2 hits per bin on average: 1 - e^-2 = 86 % of bins are reached
 
 
 def test_full_size_cfg2_conservation(rf, gpu_available):
