@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-probe", action="store_true")
+    ap.add_argument("--no-psnr", action="store_true", help="skip the time-to-40-dB measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -62,6 +63,68 @@ def cpu_baseline(wl, budget_s):
             "sample": f"{wl.name} flame, FP64 C oracle (per-particle chaos game), {parts} particles x "
                       f"({fuse} unbinned + {iters} binned) iterations on {threads} threads into a "
                       f"{W // 4}x{H // 4} histogram; {n} binned in {dt:.1f} s"}
+
+
+def time_to_psnr(rf, wl, target_db, hist_mode, waves):
+    """Plain method (more samples + DE): wall time from a fresh flame until the
+    normalised log-density image of the density-estimated histogram reaches
+    target_db against the FP64 oracle's histogram image at 2^32 samples
+    (tests/golden/psnr_<cfg>.npz, `raw`).  Samples double from one pass over
+    the particles up to the workload's whole job; the clock covers warmup,
+    draws and the DE of the image that is scored, not the scoring."""
+    import torch
+    from refrakt_amd import quality
+    golden = os.path.join(ROOT, "tests", "golden")
+    try:
+        meta = json.load(open(os.path.join(golden, f"psnr_{wl.name}.json")))
+        ref = quality.dequantize(np.load(os.path.join(golden, f"psnr_{wl.name}.npz"))["raw"], meta["peak_raw"])
+    except OSError:
+        return None
+    ref_t = torch.from_numpy(ref).cuda()
+    ctx = rf.Context(wl.particles, wl.temporal_samples, 1024, seed=777, engine=rf.RF_ENGINE_FAST,
+                     hist_mode=hist_mode, block_waves=waves)
+    fh = ctx.flame(wl.flame)
+    hist_t = torch.zeros((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
+    hist = ctx.histogram(wl.width, wl.height, device_ptr=hist_t.data_ptr())
+    de = torch.empty((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
+    f = wl.flame
+    dep = rf.DEParams(f.estimator_radius, f.estimator_min, f.estimator_curve)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fh.warmup(wl.warmup_passes)
+    torch.cuda.synchronize()
+    draw_clock = time.perf_counter() - t0      # warmup + every draw so far
+    total_passes = wl.draw_passes * wl.draw_calls
+    done, drawn, curve, hit, hit_raw = 0, 0, [], None, None
+    while done < total_passes:
+        n = min(max(done, 1), total_passes - done)   # doubles the samples
+        t0 = time.perf_counter()
+        drawn += fh.draw_to_bins(hist, n)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.density_estimate(hist, dep, de.data_ptr())
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        done += n
+        draw_clock += t1 - t0
+        wall = draw_clock + (t2 - t1)            # one DE of the image that is scored
+        p = quality.psnr(quality.log_density(quality.image_to_bin_layout(de), drawn, meta["pool"]), ref_t)
+        p_raw = quality.psnr(quality.log_density(hist_t, drawn, meta["pool"]), ref_t)
+        curve.append({"samples": drawn, "psnr_db": round(p, 2), "wall_s": round(wall, 4),
+                      "psnr_raw_db": round(p_raw, 2), "wall_raw_s": round(draw_clock, 4)})
+        if hit is None and p >= target_db:
+            hit = curve[-1]
+        if hit_raw is None and p_raw >= target_db:
+            hit_raw = curve[-1]
+    hist.close()
+    ctx.close()
+    return {"target_db": target_db, "plain_s": hit["wall_s"] if hit else None,
+            "plain_samples": hit["samples"] if hit else None, "psnr_full_job_db": curve[-1]["psnr_db"],
+            "raw_s": hit_raw["wall_raw_s"] if hit_raw else None,
+            "raw_samples": hit_raw["samples"] if hit_raw else None,
+            "psnr_raw_full_job_db": curve[-1]["psnr_raw_db"], "denoised_s": None,
+            "reference": f"FP64 oracle histogram, {meta['binned']} binned samples, pool {meta['pool']}",
+            "curve": curve}
 
 
 def main():
@@ -166,6 +229,8 @@ def main():
         "roofline": roofline,
         "de_tonemap_ms": round(de_ms, 3),
     }
+    if not args.no_psnr:
+        line["time_to_40db"] = time_to_psnr(rf, wl, 40.0, args.hist_mode, args.waves)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(line))

@@ -246,6 +246,17 @@ int rf_density_tonemap(rf_ctx* ctx, rf_hist* bins, const rf_de_params* de,
 int rf_render(rf_ctx* ctx, rf_flame* flame, const rf_render_params* params,
               float* hist_out, float* image_out, rf_stats* stats);
 
+/* ---------------------------------------------------------- output */
+/* RGBA float -> RGBA8 the way the reference reads its render target back for
+ * a screenshot (src/buffer_objects.hpp:117: clamp to [0,1], x255, round to
+ * nearest; NaN -> 0).  Host arrays of W*H*4. */
+int rf_rgba8_from_f32(const float* in, uint32_t width, uint32_t height, uint8_t* out);
+/* The same conversion on the GPU: device pointers, on the context's stream. */
+int rf_rgba8_device(rf_ctx* ctx, const void* in_device, void* out_device, uint32_t width, uint32_t height);
+/* Writes W*H RGBA8 pixels (first row first) as a PNG file, ≙ the screenshot
+ * of src/main.cpp:590-593. */
+int rf_write_png(const char* path, const uint8_t* rgba8, uint32_t width, uint32_t height);
+
 /* ---------------------------------------------------- state inspection */
 /* Particle buffer (P x (x, y, color, 0)) after the last pass. */
 int rf_particles_download(rf_ctx* ctx, rf_flame* flame, float* host);

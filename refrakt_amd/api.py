@@ -119,6 +119,9 @@ _SIGS = {
     "rf_shuffle_download": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "rf_sample_points_download": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "rf_flame_hist_mode": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "rf_rgba8_from_f32": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
+    "rf_rgba8_device": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "rf_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),
     "rf_screen_affine": (C.c_int, [C.POINTER(FlameDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     "rf_variation_count": (C.c_int, []),
     "rf_variation_name": (C.c_char_p, [C.c_int]),
@@ -288,6 +291,28 @@ class Flame:
                      brightness=d.brightness)
 
 
+def to_rgba8(img: np.ndarray) -> np.ndarray:
+    """(H, W, 4) float image -> (H, W, 4) uint8, the screenshot conversion."""
+    img = np.ascontiguousarray(img, np.float32)
+    if img.ndim != 3 or img.shape[2] != 4:
+        raise ValueError("expected an (H, W, 4) image")
+    out = np.empty(img.shape, np.uint8)
+    rc = lib().rf_rgba8_from_f32(img.ctypes.data, img.shape[1], img.shape[0], out.ctypes.data)
+    if rc != RF_OK:
+        raise RefraktError(f"rf_rgba8_from_f32 failed ({rc})")
+    return out
+
+
+def write_png(path: str, rgba8: np.ndarray) -> None:
+    """Writes an (H, W, 4) uint8 image as a PNG (first row at the top)."""
+    rgba8 = np.ascontiguousarray(rgba8, np.uint8)
+    if rgba8.ndim != 3 or rgba8.shape[2] != 4:
+        raise ValueError("expected an (H, W, 4) uint8 image")
+    rc = lib().rf_write_png(os.fsencode(path), rgba8.ctypes.data, rgba8.shape[1], rgba8.shape[0])
+    if rc != RF_OK:
+        raise RefraktError(f"rf_write_png failed ({rc}) for {path}")
+
+
 def param_block(flame: Flame) -> np.ndarray:
     """The reference's flat parameter block (src/flame.cpp:33-103)."""
     d = flame.to_c()
@@ -382,8 +407,9 @@ class Context:
     def flame(self, flame: Flame) -> "FlameHandle":
         return FlameHandle(self, flame)
 
-    def histogram(self, width: int, height: int) -> "Histogram":
-        return Histogram(self, width, height)
+    def histogram(self, width: int, height: int, device_ptr: Optional[int] = None) -> "Histogram":
+        """A W x H RGBA float histogram; with device_ptr, wraps caller-owned device memory."""
+        return Histogram(self, width, height, device_ptr)
 
     def shuffle_rows(self) -> np.ndarray:
         ppt = self.total_particles // self.temporal_samples
@@ -430,6 +456,10 @@ class Context:
 
     def tonemap(self, in_ptr: int, out_ptr: int, width: int, height: int, tm: TonemapParams):
         self.check(self.L.rf_tonemap(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), width, height, C.byref(tm)))
+
+    def rgba8(self, in_ptr: int, out_ptr: int, width: int, height: int):
+        """RGBA float image -> RGBA8 on the GPU (device pointers)."""
+        self.check(self.L.rf_rgba8_device(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), width, height))
 
     def render(self, fh: "FlameHandle", params: RenderParams, want_hist=True, want_image=True):
         W, H = params.width, params.height

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librefrakt_amd.so")
-SOURCES = ["rf_k_compat.hip", "rf_k_fast.hip", "rf_k_misc.hip", "rf_host.cpp", "rf_flam3.cpp"]
+SOURCES = ["rf_k_compat.hip", "rf_k_fast.hip", "rf_k_misc.hip", "rf_host.cpp", "rf_flam3.cpp", "rf_output.cpp"]
 ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -65,7 +65,7 @@ def build(verbose=False):
                 print(o)
     objs = [j[1] for j in jobs]
     tmp = LIB + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
     os.replace(tmp, LIB)
     return LIB
 

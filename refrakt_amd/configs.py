@@ -13,6 +13,11 @@ cfg2  full size: 8 xforms + a final xform, post-affines, 3-5 variations per
 cfg3  atomic contention: 4 strongly contractive xforms (scale U[0.05,0.15],
       offsets within 0.1 of the origin); 8192x8192 accumulator, 2^26
       particles, 2^34 binned samples.
+cfg4  batching: 64 animation frames of one seeded flame family; frame i at
+      t = i/63 uses the family's first n_i xforms (n_i uneven in 2..12) with
+      every affine rotated by 2 pi t times the xform's rotation rate, an
+      accumulator of 512^2, 1024^2, 1920x1080 or 2048^2 and 64..512 samples
+      per pixel.
 
 Every flame is a pure function of its seed (numpy PCG64).
 """
@@ -20,6 +25,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
+from typing import List
 
 import numpy as np
 
@@ -118,6 +124,56 @@ def cfg3(seed: int = 3) -> Workload:
               rotate=0.0, estimator_min=0, estimator_radius=9, estimator_curve=0.4, gamma=2.2, vibrancy=1.0,
               brightness=4.0)
     return Workload("cfg3", f, 8192, 8192, 1 << 26, 512, 19, 16, 16)
+
+
+@dataclass
+class Frame:
+    flame: Flame
+    width: int
+    height: int
+    spp: int
+
+    @property
+    def samples(self) -> int:
+        return self.spp * self.width * self.height
+
+
+CFG4_SIZES = [(512, 512), (1024, 1024), (1920, 1080), (2048, 2048)]
+
+
+def cfg4(seed: int = 4, frames: int = 64) -> List[Frame]:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    fam = []
+    for _ in range(12):
+        k = int(rng.integers(2, 4))
+        names = list(rng.choice(CFG2_VARIATIONS, size=k, replace=False))
+        w = rng.uniform(0.2, 1, size=k)
+        fam.append({"aff": _rot_scale(rng, 0.35, 0.85) + list(rng.uniform(-0.8, 0.8, 2)),
+                    "vars": {n: float(v) for n, v in zip(names, w / w.sum())},
+                    "weight": float(rng.uniform(0.2, 1.0)), "color": float(rng.uniform(0, 1)),
+                    "rate": float(rng.choice([-1.0, 1.0]) * rng.uniform(0.5, 1.0))})
+    pal = smooth_palette(rng)
+    out = []
+    for i in range(frames):
+        t = i / max(1, frames - 1)
+        n = int(rng.integers(2, 13))
+        W, H = CFG4_SIZES[int(rng.integers(0, len(CFG4_SIZES)))]
+        spp = int(rng.integers(64, 513))
+        xs = []
+        for x in fam[:n]:
+            th = 2 * math.pi * t * x["rate"]
+            c, s_ = math.cos(th), math.sin(th)
+            a, b, cc, d, e, f = x["aff"]
+            # rotate the linear part: [a c; b d] <- R(th) [a c; b d]
+            ra, rb = c * a - s_ * b, s_ * a + c * b
+            rc, rd = c * cc - s_ * d, s_ * cc + c * d
+            xs.append(Xform(affine=[ra, rb, rc, rd, e, f], variations=dict(x["vars"]), weight=x["weight"],
+                            color=x["color"], color_speed=0.5, opacity=1.0))
+        fl = Flame(xforms=xs, palette=pal, size=[W, H], center=[0.0, 0.0], scale=min(W, H) / 4.0, rotate=0.0,
+                   estimator_min=0, estimator_radius=9, estimator_curve=0.4, gamma=2.2, vibrancy=1.0,
+                   brightness=4.0)
+        out.append(Frame(fl, W, H, spp))
+    return out
 
 
 WORKLOADS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3}

@@ -685,6 +685,16 @@ int rf_flame_validate(const rf_flame_desc* d, char* err, size_t err_len) {
   return ok ? RF_OK : RF_ERR_ARG;
 }
 
+int rf_rgba8_device(rf_ctx* ctx, const void* in, void* out, uint32_t w, uint32_t h) {
+  if (!ctx) return RF_ERR_ARG;
+  if (!in || !out || w == 0 || h == 0) return set_err(ctx, RF_ERR_ARG, "rgba8: null pointer or empty image");
+  const uint64_t n = (uint64_t)w * h;
+  if (n > 0xffffffffull) return set_err(ctx, RF_ERR_LIMIT, "rgba8: more than 2^32 pixels");
+  HIPCHK(ctx, rfk::launch_rgba8((const float4*)in, (uchar4*)out, (uint32_t)n, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return RF_OK;
+}
+
 int rf_flame_hist_mode(const rf_flame* f, double* frac) {
   if (!f) return RF_ERR_ARG;
   if (frac) *frac = f->last_lds_hit_fraction;

@@ -64,6 +64,7 @@ hipError_t launch_animate(rfd::Block* blocks, uint32_t T, float tss_width, const
 hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s);
 hipError_t launch_tonemap(const float4* in, float4* out, uint32_t n, float gamma, float brightness,
                           float vibrancy, float k, hipStream_t s);
+hipError_t launch_rgba8(const float4* in, uchar4* out, uint32_t n, hipStream_t s);
 hipError_t launch_eval_xform(const rfd::Block* blk, const float* pts, const float* rands, float* out,
                              uint32_t n, hipStream_t s);
 hipError_t launch_atomic_probe(float* bins, uint64_t nbins, uint64_t samples, int hist_mode,

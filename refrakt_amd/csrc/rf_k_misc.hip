@@ -182,6 +182,22 @@ __global__ void k_tonemap(const float4* __restrict__ in, float4* __restrict__ ou
   if (i < n) out[i] = tonemap_px(in[i], gamma, brightness, vib, k);
 }
 
+// RGBA float -> RGBA8: clamp to [0, 1], x255, round to nearest (NaN -> 0);
+// one thread per pixel, 16 bytes in, 4 bytes out.
+__global__ void k_rgba8(const float4* __restrict__ in, uchar4* __restrict__ out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = in[i];
+  const float c[4] = {v.x, v.y, v.z, v.w};
+  uint8_t o[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float t = c[k] > 0.0f ? (c[k] < 1.0f ? c[k] : 1.0f) : 0.0f;
+    o[k] = (uint8_t)(t * 255.0f + 0.5f);
+  }
+  out[i] = make_uchar4(o[0], o[1], o[2], o[3]);
+}
+
 // ------------------------------------------------------------ diagnostics
 __global__ void k_eval_xform(const Block* __restrict__ blk, const float* __restrict__ pts,
                              const float* __restrict__ rands, float* __restrict__ out, uint32_t n) {
@@ -238,6 +254,11 @@ hipError_t launch_tonemap(const float4* in, float4* out, uint32_t n, float gamma
                           float vibrancy, float k, hipStream_t s) {
   hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n, gamma, brightness,
                      vibrancy, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_rgba8(const float4* in, uchar4* out, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_rgba8, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n);
   return hipGetLastError();
 }
 

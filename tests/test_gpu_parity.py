@@ -348,3 +348,15 @@ def test_full_size_cfg2_conservation(rf, gpu_available):
     assert int(g[..., 3].astype(np.float64).sum()) == drawn
     pos = fh.particles()
     assert np.isfinite(pos).all()
+
+
+def test_rgba8_device_matches_host(rf, gpu_available):
+    import torch
+    ctx = _ctx(rf, 1 << 10, 1, rf.RF_ENGINE_FAST)
+    rng = np.random.default_rng(9)
+    img = rng.uniform(-0.2, 1.2, size=(67, 45, 4)).astype(np.float32)
+    img[3, 4, 1] = np.nan
+    a = torch.from_numpy(img).cuda()
+    b = torch.empty((67, 45, 4), dtype=torch.uint8, device="cuda")
+    ctx.rgba8(a.data_ptr(), b.data_ptr(), 45, 67)
+    assert np.array_equal(b.cpu().numpy(), rf.to_rgba8(img))

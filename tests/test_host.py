@@ -71,3 +71,41 @@ def test_configs_are_deterministic(rf):
     assert len(c2.xforms) == 8 and c2.final_xform is not None
     assert all(x.post is not None and 3 <= len(x.variations) <= 5 for x in c2.xforms)
     assert all(abs(sum(x.variations.values()) - 1) < 1e-6 for x in c2.xforms)
+
+
+def _read_png(path):
+    """Minimal PNG reader for 8-bit RGBA, non-interlaced files with filter-0 rows."""
+    import struct
+    import zlib
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, ihdr = 8, b"", None
+    while pos < len(data):
+        n, typ = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + n]
+        crc = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])[0]
+        assert zlib.crc32(typ + body) & 0xffffffff == crc, typ
+        if typ == b"IHDR":
+            ihdr = struct.unpack(">IIBBBBB", body)
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    w, h, depth, ctype, comp, filt, inter = ihdr
+    assert (depth, ctype, comp, filt, inter) == (8, 6, 0, 0, 0)
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 4 * w)
+    assert (raw[:, 0] == 0).all()
+    return raw[:, 1:].reshape(h, w, 4)
+
+
+def test_png_round_trip_and_rgba8_conversion(rf, tmp_path):
+    rng = np.random.default_rng(5)
+    img = rng.uniform(-0.2, 1.2, size=(37, 53, 4)).astype(np.float32)
+    img[0, 0, 0] = np.nan
+    img[0, 1, :] = [0.5 / 255, 1.5 / 255, 254.5 / 255, 1.0]
+    q = rf.to_rgba8(img)
+    ref = np.floor(np.clip(np.nan_to_num(img, nan=0.0), 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.uint8)
+    assert np.array_equal(q, ref)
+    assert q[0, 0, 0] == 0 and q[0, 1, 3] == 255
+    path = str(tmp_path / "x.png")
+    rf.write_png(path, q)
+    assert np.array_equal(_read_png(path), q)
