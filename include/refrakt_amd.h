@@ -246,6 +246,51 @@ int rf_density_tonemap(rf_ctx* ctx, rf_hist* bins, const rf_de_params* de,
 int rf_render(rf_ctx* ctx, rf_flame* flame, const rf_render_params* params,
               float* hist_out, float* image_out, rf_stats* stats);
 
+/* ---------------------------------------------------------- batching */
+/* Many frames (each its own flame, size and sample count) rendered by one
+ * persistent chaos-game launch followed by every frame's DE + tonemap, all
+ * captured once in a HIP graph and replayed by rf_batch_run.  This
+ * generalises the reference's temporal-sample axis (one dispatch over T
+ * parameter blocks, src/flame.cpp:275, shaders/templates/animate.tpl.glsl)
+ * to frames of uneven size and xform count.  No reference counterpart. */
+typedef struct rf_batch rf_batch;
+
+typedef struct rf_frame_desc {
+  const rf_flame_desc* flame;  /* read during rf_batch_create only */
+  uint32_t width, height;      /* accumulator and image size */
+  uint64_t samples;            /* binned samples wanted, e.g. spp * width * height */
+} rf_frame_desc;
+
+typedef struct rf_batch_config {
+  int32_t block_waves;         /* 4 or 8: a work item is 64 * block_waves particles */
+  int32_t hist_mode;           /* RF_HIST_QUAD or RF_HIST_LDS */
+  uint32_t iters;              /* binned iterations per particle */
+  uint32_t fuse;               /* unbinned iterations per particle before those */
+} rf_batch_config;
+
+int rf_batch_create(rf_ctx* ctx, const rf_frame_desc* frames, uint32_t n_frames, rf_batch** out);
+void rf_batch_destroy(rf_batch* batch);
+/* Sets the launch shape and captures the graph (default: 8 waves, LDS, 256, 20). */
+int rf_batch_set_config(rf_batch* batch, const rf_batch_config* cfg);
+int rf_batch_get_config(const rf_batch* batch, rf_batch_config* cfg);
+/* Autotuner: times every candidate shape (block_waves x hist_mode x iters) on
+ * a copy of the batch whose sample counts are scaled by sample_fraction
+ * (0 < f <= 1), then sets the fastest.  best/best_ms may be NULL. */
+int rf_batch_tune(rf_batch* batch, double sample_fraction, rf_batch_config* best, double* best_ms);
+/* Replays the graph: zero, chaos game, DE + tonemap of every frame; *ms is
+ * the host wall time of the replay. */
+int rf_batch_run(rf_batch* batch, double* ms);
+/* Per-frame results of the last run (arrays of n_frames, NULL to skip):
+ * binned samples, and the device-clock times, in ms from the start of the
+ * replay, at which the frame's last samples reached its histogram and its
+ * tonemapped image was complete. */
+int rf_batch_frame_stats(const rf_batch* batch, uint64_t* drawn, double* chaos_done_ms, double* image_done_ms);
+/* Device pointers of frame i's histogram (bins layout) and tonemapped image. */
+void* rf_batch_hist_ptr(rf_batch* batch, uint32_t i);
+void* rf_batch_image_ptr(rf_batch* batch, uint32_t i);
+/* Copies frame i's histogram (which 0) or image (which 1) to host (W*H*4 floats). */
+int rf_batch_download(rf_batch* batch, uint32_t i, int32_t which, float* host);
+
 /* ---------------------------------------------------------- output */
 /* RGBA float -> RGBA8 the way the reference reads its render target back for
  * a screenshot (src/buffer_objects.hpp:117: clamp to [0,1], x255, round to
@@ -298,7 +343,8 @@ int rf_atomic_probe(rf_ctx* ctx, int32_t hist_mode, uint64_t footprint_bins,
 
 /* sizeof of the ABI structs (0 rf_var_term, 1 rf_xform, 2 rf_flame_desc,
  * 3 rf_sim_params, 4 rf_de_params, 5 rf_tonemap_params, 6 rf_render_params,
- * 7 rf_stats), for bindings that check their layouts. */
+ * 7 rf_stats, 8 rf_frame_desc, 9 rf_batch_config), for bindings that check
+ * their layouts. */
 size_t rf_struct_size(int which);
 
 #ifdef __cplusplus

@@ -77,7 +77,17 @@ class Stats(C.Structure):
                 ("ms_draw", C.c_double), ("ms_density", C.c_double), ("ms_tonemap", C.c_double)]
 
 
-_STRUCTS = [VarTerm, XformDesc, FlameDesc, SimParams, DEParams, TonemapParams, RenderParams, Stats]
+class FrameDescC(C.Structure):
+    _fields_ = [("flame", C.POINTER(FlameDesc)), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("samples", C.c_uint64)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("block_waves", C.c_int32), ("hist_mode", C.c_int32), ("iters", C.c_uint32), ("fuse", C.c_uint32)]
+
+
+_STRUCTS = [VarTerm, XformDesc, FlameDesc, SimParams, DEParams, TonemapParams, RenderParams, Stats, FrameDescC,
+            BatchConfig]
 
 _P = C.c_void_p
 _SIGS = {
@@ -119,6 +129,16 @@ _SIGS = {
     "rf_shuffle_download": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "rf_sample_points_download": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "rf_flame_hist_mode": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "rf_batch_create": (C.c_int, [_P, C.POINTER(FrameDescC), C.c_uint32, C.POINTER(_P)]),
+    "rf_batch_destroy": (None, [_P]),
+    "rf_batch_set_config": (C.c_int, [_P, C.POINTER(BatchConfig)]),
+    "rf_batch_get_config": (C.c_int, [_P, C.POINTER(BatchConfig)]),
+    "rf_batch_tune": (C.c_int, [_P, C.c_double, C.POINTER(BatchConfig), C.POINTER(C.c_double)]),
+    "rf_batch_run": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "rf_batch_frame_stats": (C.c_int, [_P, _P, _P, _P]),
+    "rf_batch_hist_ptr": (_P, [_P, C.c_uint32]),
+    "rf_batch_image_ptr": (_P, [_P, C.c_uint32]),
+    "rf_batch_download": (C.c_int, [_P, C.c_uint32, C.c_int32, _P]),
     "rf_rgba8_from_f32": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
     "rf_rgba8_device": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "rf_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),
@@ -461,6 +481,10 @@ class Context:
         """RGBA float image -> RGBA8 on the GPU (device pointers)."""
         self.check(self.L.rf_rgba8_device(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), width, height))
 
+    def batch(self, frames) -> "Batch":
+        """Batched rendering of (flame, width, height, samples) frames."""
+        return Batch(self, frames)
+
     def render(self, fh: "FlameHandle", params: RenderParams, want_hist=True, want_image=True):
         W, H = params.width, params.height
         hist = np.zeros((H, W, 4), np.float32) if want_hist else None
@@ -590,3 +614,77 @@ def render(ctx: Context, flame: Flame, width: int, height: int, warmup_passes: i
         return ctx.render(fh, rp)
     finally:
         fh.close()
+
+
+class Batch:
+    """Many frames in one persistent chaos-game launch plus per-frame DE and
+    tonemap, captured in a HIP graph (rf_batch_*)."""
+
+    def __init__(self, ctx: Context, frames):
+        self.ctx = ctx
+        self.sizes = [(int(w), int(h)) for _, w, h, _ in frames]
+        descs = [f.to_c() for f, _, _, _ in frames]
+        arr = (FrameDescC * len(frames))()
+        for i, (d, (_, w, h, n)) in enumerate(zip(descs, frames)):
+            arr[i].flame = C.pointer(d)
+            arr[i].width, arr[i].height, arr[i].samples = int(w), int(h), int(n)
+        h = _P()
+        ctx.check(ctx.L.rf_batch_create(ctx.h, arr, len(frames), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.rf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_config(self, block_waves=8, hist_mode=RF_HIST_LDS, iters=256, fuse=20):
+        cfg = BatchConfig(block_waves, hist_mode, iters, fuse)
+        self.ctx.check(self.ctx.L.rf_batch_set_config(self.h, C.byref(cfg)))
+
+    def config(self) -> dict:
+        cfg = BatchConfig()
+        self.ctx.check(self.ctx.L.rf_batch_get_config(self.h, C.byref(cfg)))
+        return {"block_waves": cfg.block_waves, "hist_mode": cfg.hist_mode, "iters": cfg.iters, "fuse": cfg.fuse}
+
+    def tune(self, sample_fraction: float = 0.125):
+        cfg, ms = BatchConfig(), C.c_double()
+        self.ctx.check(self.ctx.L.rf_batch_tune(self.h, sample_fraction, C.byref(cfg), C.byref(ms)))
+        return self.config(), ms.value
+
+    def run(self) -> float:
+        ms = C.c_double()
+        self.ctx.check(self.ctx.L.rf_batch_run(self.h, C.byref(ms)))
+        return ms.value
+
+    def frame_stats(self) -> dict:
+        n = len(self.sizes)
+        drawn = np.zeros(n, np.uint64)
+        chaos = np.zeros(n, np.float64)
+        image = np.zeros(n, np.float64)
+        self.ctx.check(self.ctx.L.rf_batch_frame_stats(self.h, drawn.ctypes.data, chaos.ctypes.data,
+                                                       image.ctypes.data))
+        return {"drawn": drawn, "chaos_done_ms": chaos, "image_done_ms": image}
+
+    def _download(self, i: int, which: int) -> np.ndarray:
+        w, h = self.sizes[i]
+        out = np.zeros((h, w, 4), np.float32)
+        self.ctx.check(self.ctx.L.rf_batch_download(self.h, i, which, out.ctypes.data))
+        return out
+
+    def histogram(self, i: int) -> np.ndarray:
+        return self._download(i, 0)
+
+    def image(self, i: int) -> np.ndarray:
+        return self._download(i, 1)
+
+    def hist_ptr(self, i: int) -> int:
+        return self.ctx.L.rf_batch_hist_ptr(self.h, i)
+
+    def image_ptr(self, i: int) -> int:
+        return self.ctx.L.rf_batch_image_ptr(self.h, i)

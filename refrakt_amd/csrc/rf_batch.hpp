@@ -1,0 +1,40 @@
+// Batched rendering: device-side frame table and kernel arguments.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rf_device.hpp"
+
+namespace rfk {
+
+struct BatchFrame {
+  float* bins;            // W*H RGBA, rows bottom-up
+  uint32_t W, H;
+  uint32_t item_begin;    // first work item of the frame (frames are sorted by it)
+  uint32_t n_items;
+  float ss[6];            // screen affine
+  uint32_t pad[2];
+};
+static_assert(sizeof(BatchFrame) == 56, "BatchFrame layout");
+
+struct BatchArgs {
+  const BatchFrame* frames;
+  const rfd::Block* blocks;       // one per frame
+  const float4* palettes;         // 256 per frame
+  const uint16_t* perms;          // NPERM rows x (64*waves)
+  unsigned long long* drawn;      // per frame: binned samples
+  uint32_t* remaining;            // per frame: items whose samples are not yet in the histogram
+  unsigned long long* done_ts;    // per frame: wall clock when the last item was delivered
+  uint64_t seed;
+  uint32_t launch;
+  uint32_t n_frames, n_items;
+  uint32_t fuse, n_iter;          // unbinned and binned iterations per particle
+};
+
+// Caches the resident-workgroup count of a kernel variant (call before a capture).
+hipError_t batch_prepare(int waves, int hist_mode);
+hipError_t launch_batch(const BatchArgs& a, int waves, int hist_mode, uint32_t max_grid, hipStream_t s);
+hipError_t launch_batch_init(const BatchArgs& a, float4* hist, uint64_t n_bins, const uint32_t* items,
+                             uint32_t* tiles_done, unsigned long long* t0, hipStream_t s);
+
+}  // namespace rfk

@@ -232,16 +232,21 @@ void screen_affine(const rf_flame_desc& d, uint32_t w, uint32_t h, float out[6])
 using namespace rfh;
 
 // ---------------------------------------------------------------- helpers
-static int set_err(rf_ctx* ctx, int code, const std::string& msg) {
+int rfh::set_err(rf_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   return code;
 }
-#define HIPCHK(ctx, expr)                                                                   \
-  do {                                                                                      \
-    hipError_t _e = (expr);                                                                 \
-    if (_e != hipSuccess)                                                                   \
-      return set_err((ctx), RF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-  } while (0)
+
+std::vector<uint16_t> rfh::perm_rows(uint64_t seed, int waves) {
+  const uint32_t N = 64u * (uint32_t)waves;
+  std::vector<uint16_t> tab((size_t)rfk::NPERM * N);
+  std::vector<uint32_t> row(N);
+  for (int r = 0; r < rfk::NPERM; ++r) {
+    shuffle_row(row.data(), N, seed ^ 0x5EED5EED00000000ull ^ (uint64_t)(r + 1) * 0x2545F4914F6CDD1Dull);
+    for (uint32_t i = 0; i < N; ++i) tab[(size_t)r * N + i] = (uint16_t)row[i];
+  }
+  return tab;
+}
 
 static int validate_sim(rf_ctx* ctx, uint64_t P, uint32_t T) {
   if (T == 0) return set_err(ctx, RF_ERR_ARG, "temporal_samples must be positive");
@@ -303,13 +308,7 @@ static int fast_waves(const rf_ctx* ctx) {
 static int ensure_perms(rf_ctx* ctx) {
   const int waves = fast_waves(ctx);
   if (ctx->perms.p && ctx->perm_waves == waves) return RF_OK;
-  const uint32_t N = 64u * (uint32_t)waves;
-  std::vector<uint16_t> tab((size_t)rfk::NPERM * N);
-  std::vector<uint32_t> row(N);
-  for (int r = 0; r < rfk::NPERM; ++r) {
-    shuffle_row(row.data(), N, ctx->sp.seed ^ 0x5EED5EED00000000ull ^ (uint64_t)(r + 1) * 0x2545F4914F6CDD1Dull);
-    for (uint32_t i = 0; i < N; ++i) tab[(size_t)r * N + i] = (uint16_t)row[i];
-  }
+  const std::vector<uint16_t> tab = perm_rows(ctx->sp.seed, waves);
   HIPCHK(ctx, ctx->perms.alloc(tab.size() * 2));
   HIPCHK(ctx, hipMemcpy(ctx->perms.p, tab.data(), tab.size() * 2, hipMemcpyHostToDevice));
   ctx->perm_waves = waves;
@@ -911,6 +910,8 @@ size_t rf_struct_size(int which) {
     case 5: return sizeof(rf_tonemap_params);
     case 6: return sizeof(rf_render_params);
     case 7: return sizeof(rf_stats);
+    case 8: return sizeof(rf_frame_desc);
+    case 9: return sizeof(rf_batch_config);
     default: return 0;
   }
 }

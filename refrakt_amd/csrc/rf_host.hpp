@@ -9,6 +9,8 @@
 #include "../../include/refrakt_amd.h"
 #include "rf_device.hpp"
 
+struct rf_ctx;
+
 struct rf_var_info {
   const char* name;
   int n_params;
@@ -45,8 +47,19 @@ bool build_block(const rf_flame_desc& d, rfd::Block& b, std::string& err);
 std::vector<float> reference_param_block(const rf_flame_desc& d);
 // Screen-space affine (src/flame.cpp:289-296, src/flame.hpp:97-128).
 void screen_affine(const rf_flame_desc& d, uint32_t w, uint32_t h, float out[6]);
+// The fast engine's 16 LDS permutation rows for 64*waves lanes.
+std::vector<uint16_t> perm_rows(uint64_t seed, int waves);
+// Records msg as the context's last error and returns code.
+int set_err(rf_ctx* ctx, int code, const std::string& msg);
 
 }  // namespace rfh
+
+#define HIPCHK(ctx, expr)                                                                         \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess)                                                                         \
+      return rfh::set_err((ctx), RF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
 
 struct rf_ctx {
   rf_sim_params sp{};

@@ -53,6 +53,10 @@ struct DEArgs {
   float curve;
   int32_t do_tonemap;
   float gamma, brightness, vibrancy, k;
+  // Optional completion stamp (batched rendering): the last workgroup to
+  // finish writes the wall clock to *done_ts; *tiles_done counts workgroups.
+  uint32_t* tiles_done;
+  unsigned long long* done_ts;
 };
 
 constexpr int NPERM = 16;         // permutation rows of the fast engine (held in LDS)

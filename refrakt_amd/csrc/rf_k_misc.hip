@@ -118,6 +118,21 @@ __device__ __forceinline__ float4 tonemap_px(float4 c, float gamma, float bright
   return make_float4(o[0], o[1], o[2], 1.0f);
 }
 
+// Batched rendering: the last workgroup of a frame's DE stamps its completion.
+// Every thread reaches this point (no early return), so the barrier is safe.
+__device__ __forceinline__ void de_stamp(const DEArgs& g) {
+  if (!g.done_ts) return;  // kernel argument: uniform
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t n = gridDim.x * gridDim.y;
+    if (atomicAdd(g.tiles_done, 1u) + 1u == n) {
+      const unsigned long long t = wall_clock64();
+      __hip_atomic_store(g.done_ts, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 __global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
   extern __shared__ unsigned char smem[];
   const int R = g.R;
@@ -136,44 +151,48 @@ __global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
   __syncthreads();
   const int lx = tid % DE_TW, ly = tid / DE_TW;
   const int tx = tx0 + lx, ty = ty0 + ly;
-  if (tx >= (int)g.W || ty >= (int)g.H) return;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int i = -R; i <= R; ++i) {
-    // source row cy = ty - i -> LDS row ly + R - i
-    const int row = (ly + R - i) * SW;
-    for (int j = -R; j <= R; ++j) {
-      const int k = row + lx + R - j;
-      const DESrc m = s_m[k];
-      const float wgt = de_weight(m, i, j);
-      if (wgt != 0.0f) {
-        const float4 c = s_col[k];
-        acc.x += c.x * wgt; acc.y += c.y * wgt; acc.z += c.z * wgt; acc.w += c.w * wgt;
+  if (tx < (int)g.W && ty < (int)g.H) {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i = -R; i <= R; ++i) {
+      // source row cy = ty - i -> LDS row ly + R - i
+      const int row = (ly + R - i) * SW;
+      for (int j = -R; j <= R; ++j) {
+        const int k = row + lx + R - j;
+        const DESrc m = s_m[k];
+        const float wgt = de_weight(m, i, j);
+        if (wgt != 0.0f) {
+          const float4 c = s_col[k];
+          acc.x += c.x * wgt; acc.y += c.y * wgt; acc.z += c.z * wgt; acc.w += c.w * wgt;
+        }
       }
     }
+    const size_t o = (size_t)ty * g.W + tx;
+    if (g.de_out) g.de_out[o] = acc;
+    if (g.img_out && g.do_tonemap) g.img_out[o] = tonemap_px(acc, g.gamma, g.brightness, g.vibrancy, g.k);
   }
-  const size_t o = (size_t)ty * g.W + tx;
-  if (g.de_out) g.de_out[o] = acc;
-  if (g.img_out && g.do_tonemap) g.img_out[o] = tonemap_px(acc, g.gamma, g.brightness, g.vibrancy, g.k);
+  de_stamp(g);
 }
 
 // Radii beyond the LDS tile: the same gather straight from memory.
 __global__ __launch_bounds__(256) void k_de_global(DEArgs g) {
   const int tx = blockIdx.x * 16 + (threadIdx.x % 16);
   const int ty = blockIdx.y * 16 + (threadIdx.x / 16);
-  if (tx >= (int)g.W || ty >= (int)g.H) return;
-  const int R = g.R;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int i = -R; i <= R; ++i) {
-    for (int j = -R; j <= R; ++j) {
-      float4 c; DESrc m;
-      de_source(g, tx - j, ty - i, c, m);
-      const float wgt = de_weight(m, i, j);
-      if (wgt != 0.0f) { acc.x += c.x * wgt; acc.y += c.y * wgt; acc.z += c.z * wgt; acc.w += c.w * wgt; }
+  if (tx < (int)g.W && ty < (int)g.H) {
+    const int R = g.R;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i = -R; i <= R; ++i) {
+      for (int j = -R; j <= R; ++j) {
+        float4 c; DESrc m;
+        de_source(g, tx - j, ty - i, c, m);
+        const float wgt = de_weight(m, i, j);
+        if (wgt != 0.0f) { acc.x += c.x * wgt; acc.y += c.y * wgt; acc.z += c.z * wgt; acc.w += c.w * wgt; }
+      }
     }
+    const size_t o = (size_t)ty * g.W + tx;
+    if (g.de_out) g.de_out[o] = acc;
+    if (g.img_out && g.do_tonemap) g.img_out[o] = tonemap_px(acc, g.gamma, g.brightness, g.vibrancy, g.k);
   }
-  const size_t o = (size_t)ty * g.W + tx;
-  if (g.de_out) g.de_out[o] = acc;
-  if (g.img_out && g.do_tonemap) g.img_out[o] = tonemap_px(acc, g.gamma, g.brightness, g.vibrancy, g.k);
+  de_stamp(g);
 }
 
 __global__ void k_tonemap(const float4* __restrict__ in, float4* __restrict__ out, uint32_t n,

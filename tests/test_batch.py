@@ -1,0 +1,120 @@
+"""Batched rendering (rf_batch_*): many frames in one persistent launch under a
+captured HIP graph."""
+import numpy as np
+import pytest
+
+from refrakt_amd import quality
+
+pytestmark = pytest.mark.gpu
+
+
+def _flame(rf, seed, n_xf, W, H):
+    rng = np.random.default_rng(seed)
+    xs = []
+    for k in range(n_xf):
+        th = rng.uniform(0, 2 * np.pi)
+        s = rng.uniform(0.3, 0.8)
+        aff = [s * np.cos(th), s * np.sin(th), -s * np.sin(th), s * np.cos(th)] + list(rng.uniform(-0.7, 0.7, 2))
+        var = {"linear": 0.6, "spherical": 0.4} if k % 2 else {"linear": 0.5, "swirl": 0.2, "julia": 0.3}
+        xs.append(rf.Xform(affine=aff, variations=var, weight=float(rng.uniform(0.2, 1)),
+                           color=float(rng.uniform()), color_speed=0.5, opacity=1.0))
+    pal = np.ones((256, 4), np.float32)
+    pal[:, :3] = rng.uniform(0.1, 1, (256, 3))
+    return rf.Flame(xforms=xs, palette=pal, size=[W, H], center=[0.0, 0.0], scale=min(W, H) / 4.0, rotate=0.0,
+                    estimator_min=0, estimator_radius=5, estimator_curve=0.4, gamma=2.2, vibrancy=1.0,
+                    brightness=4.0)
+
+
+FRAMES = [(3, 256, 192, 2_000_000), (5, 320, 320, 3_000_000), (2, 200, 150, 1_000_000), (7, 128, 128, 500_000)]
+
+
+def _frames(rf):
+    return [(_flame(rf, 10 + i, n, W, H), W, H, S) for i, (n, W, H, S) in enumerate(FRAMES)]
+
+
+def test_batch_counts_images_and_replays(rf, gpu_available):
+    import torch
+    ctx = rf.Context(1 << 12, 1, 16, seed=5)
+    frames = _frames(rf)
+    b = ctx.batch(frames)
+    b.set_config(8, rf.RF_HIST_LDS, 64, 20)
+    b.run()
+    st = b.frame_stats()
+    hists = [b.histogram(i) for i in range(len(frames))]
+    for i, (fl, W, H, S) in enumerate(frames):
+        items = -(-S // (512 * 64))
+        d = int(st["drawn"][i])
+        assert 0.5 * S <= d <= items * 512 * 64, (i, d)
+        assert int(hists[i][..., 3].astype(np.float64).sum()) == d  # opacity 1: alpha counts samples
+        # the graph's DE + tonemap equals the single-frame call on the same histogram
+        h = ctx.histogram(W, H, device_ptr=b.hist_ptr(i))
+        out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+        ctx.density_tonemap(h, rf.DEParams(fl.estimator_radius, fl.estimator_min, fl.estimator_curve),
+                            rf.TonemapParams(fl.gamma, fl.brightness, fl.vibrancy, 1e-4), out.data_ptr())
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), b.image(i)), i
+        h.close()
+    assert (st["chaos_done_ms"] > 0).all() and (st["image_done_ms"] >= st["chaos_done_ms"]).all()
+    assert (np.diff(st["image_done_ms"]) >= 0).all()  # DE launches run in frame order
+    # a replay of the graph re-zeroes and recomputes: alpha counts are exact and equal
+    b.run()
+    st2 = b.frame_stats()
+    assert np.array_equal(st["drawn"], st2["drawn"])
+    for i in range(len(frames)):
+        h2 = b.histogram(i)
+        assert np.array_equal(h2[..., 3], hists[i][..., 3])
+        np.testing.assert_allclose(h2[..., :3], hists[i][..., :3], rtol=2e-3, atol=1e-3)
+    b.close()
+
+
+def test_batch_lds_and_quad_agree(rf, gpu_available):
+    ctx = rf.Context(1 << 12, 1, 16, seed=6)
+    frames = _frames(rf)
+    b = ctx.batch(frames)
+    out = {}
+    for mode in (rf.RF_HIST_QUAD, rf.RF_HIST_LDS):
+        b.set_config(4, mode, 128, 20)
+        b.run()
+        out[mode] = ([b.histogram(i) for i in range(len(frames))], b.frame_stats()["drawn"].copy())
+    hq, dq = out[rf.RF_HIST_QUAD]
+    hl, dl = out[rf.RF_HIST_LDS]
+    assert np.array_equal(dq, dl)
+    for a, c in zip(hq, hl):
+        assert np.array_equal(a[..., 3], c[..., 3])
+        np.testing.assert_allclose(a[..., :3], c[..., :3], rtol=2e-3, atol=1e-3)
+    b.close()
+
+
+def test_batch_frame_matches_single_frame_statistically(rf, gpu_available):
+    """A batched frame and the same flame through warmup/draw_to_bins sample the
+    same attractor: their normalised log-density images agree."""
+    fl = _flame(rf, 99, 4, 256, 256)
+    S = 1 << 24
+    ctx = rf.Context(1 << 16, 1, 16, seed=7)
+    b = ctx.batch([(fl, 256, 256, S)])
+    b.set_config(8, rf.RF_HIST_LDS, 256, 20)
+    b.run()
+    hb = b.histogram(0)
+    db = int(b.frame_stats()["drawn"][0])
+    fh = ctx.flame(fl)
+    hist = ctx.histogram(256, 256)
+    fh.warmup(19)
+    ds = sum(fh.draw_to_bins(hist, 64) for _ in range(4))
+    hs = hist.download()
+    La = quality.log_density(hb, db, 4)
+    Lb = quality.log_density(hs, ds, 4)
+    p = quality.psnr(La, Lb)
+    print(f"batched vs single-frame: PSNR {p:.2f} dB")
+    assert p > 35.0
+    b.close()
+
+
+def test_batch_autotune_picks_a_candidate(rf, gpu_available):
+    ctx = rf.Context(1 << 12, 1, 16, seed=8)
+    b = ctx.batch(_frames(rf))
+    cfg, ms = b.tune(0.5)
+    assert cfg["block_waves"] in (4, 8) and cfg["hist_mode"] in (1, 4) and cfg["iters"] in (64, 256, 1024)
+    assert ms > 0
+    b.run()
+    assert b.frame_stats()["drawn"].min() > 0
+    b.close()
