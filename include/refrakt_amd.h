@@ -291,6 +291,48 @@ void* rf_batch_image_ptr(rf_batch* batch, uint32_t i);
 /* Copies frame i's histogram (which 0) or image (which 1) to host (W*H*4 floats). */
 int rf_batch_download(rf_batch* batch, uint32_t i, int32_t which, float* host);
 
+/* ---------------------------------------------------------- denoiser */
+/* A learned stand-in for density estimation (src/main.cpp:490-515): a
+ * 4-level U-Net (channels 32-256, 1.18 M parameters, bf16 activations with
+ * fp32 accumulation on MFMA, fp32 master weights, Adam) that maps the
+ * normalised log-density image of a low-sample histogram to that of a
+ * high-sample one.  Tensors are device memory, NHWC:
+ *   x, target: N x H x W x 16 bf16 (channels 0..3 = log10(1 + v * W*H / n)
+ *              of the RGBA histogram, bins layout, from rf_log_density_input;
+ *              channels 4..15 zero), H and W multiples of 8;
+ *   y:         N x H x W x 4 float (the denoised log-density).
+ * No reference counterpart. */
+typedef struct rf_denoiser rf_denoiser;
+int rf_denoiser_create(rf_ctx* ctx, uint64_t seed, rf_denoiser** out);
+void rf_denoiser_destroy(rf_denoiser* dn);
+size_t rf_denoiser_num_params(const rf_denoiser* dn);
+int rf_denoiser_get_params(rf_denoiser* dn, float* host, size_t n);
+int rf_denoiser_set_params(rf_denoiser* dn, const float* host, size_t n);   /* also resets Adam */
+/* One Adam step on the mean squared error over channels 0..3; *loss (may be
+ * NULL, which avoids a host sync) receives the batch MSE before the step. */
+int rf_denoiser_train_step(rf_denoiser* dn, const void* x, const void* target, int32_t N, int32_t H, int32_t W,
+                           float lr, float* loss);
+int rf_denoiser_infer(rf_denoiser* dn, const void* x, float* y, int32_t N, int32_t H, int32_t W);
+/* Gradients of the MSE w.r.t. the flat fp32 parameters (host array of
+ * rf_denoiser_num_params floats), without updating them. */
+int rf_denoiser_grads(rf_denoiser* dn, const void* x, const void* target, int32_t N, int32_t H, int32_t W,
+                      float* grads, float* loss);
+/* x input of the denoiser from a device histogram (npix RGBA floats):
+ * out[p, c] = log10(1 + hist[p, c] * scale) for c < 4, 0 for c in 4..15. */
+int rf_log_density_input(rf_ctx* ctx, const void* hist, void* out, uint32_t npix, double scale);
+/* Kernel-level entry points of the network (tests).  Views: channel stride
+ * cs and first channel co of an NHWC bf16 tensor; weights bf16
+ * [3][3][Cout][Cin]; dw fp32 [3][3][Cout][Cin] and db fp32 [Cout] accumulate. */
+int rf_nn_conv3x3(rf_ctx* ctx, const void* in, int32_t in_cs, int32_t in_co, const void* w, const float* bias,
+                  void* out, int32_t out_cs, int32_t out_co, int32_t N, int32_t H, int32_t W, int32_t Cin,
+                  int32_t Cout, int32_t relu);
+int rf_nn_conv3x3_wgrad(rf_ctx* ctx, const void* x, int32_t x_cs, int32_t x_co, const void* dy, int32_t dy_cs,
+                        int32_t dy_co, float* dw, float* db, int32_t N, int32_t H, int32_t W, int32_t Cin,
+                        int32_t Cout);
+int rf_nn_flip_weights(rf_ctx* ctx, const void* w, void* wt, int32_t Cin, int32_t Cout);
+int rf_nn_adam(rf_ctx* ctx, float* p, const float* g, float* m, float* v, int32_t n, float lr, float b1, float b2,
+               float eps, int32_t step);
+
 /* ---------------------------------------------------------- output */
 /* RGBA float -> RGBA8 the way the reference reads its render target back for
  * a screenshot (src/buffer_objects.hpp:117: clamp to [0,1], x255, round to

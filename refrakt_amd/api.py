@@ -139,6 +139,26 @@ _SIGS = {
     "rf_batch_hist_ptr": (_P, [_P, C.c_uint32]),
     "rf_batch_image_ptr": (_P, [_P, C.c_uint32]),
     "rf_batch_download": (C.c_int, [_P, C.c_uint32, C.c_int32, _P]),
+    "rf_denoiser_create": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+    "rf_denoiser_destroy": (None, [_P]),
+    "rf_denoiser_num_params": (C.c_size_t, [_P]),
+    "rf_denoiser_get_params": (C.c_int, [_P, _P, C.c_size_t]),
+    "rf_denoiser_set_params": (C.c_int, [_P, _P, C.c_size_t]),
+    "rf_denoiser_train_step": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                         C.POINTER(C.c_float)]),
+    "rf_denoiser_infer": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "rf_denoiser_grads": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _P,
+                                    C.POINTER(C.c_float)]),
+    "rf_log_density_input": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double]),
+    "rf_nn_conv3x3": (C.c_int, [_P, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                C.c_int32]),
+    "rf_nn_conv3x3_wgrad": (C.c_int, [_P, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_int32,
+                                      C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int32]),
+    "rf_nn_flip_weights": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]),
+    "rf_nn_adam": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_float,
+                             C.c_float, C.c_float, C.c_int32]),
     "rf_rgba8_from_f32": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
     "rf_rgba8_device": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "rf_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),
@@ -481,6 +501,13 @@ class Context:
         """RGBA float image -> RGBA8 on the GPU (device pointers)."""
         self.check(self.L.rf_rgba8_device(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), width, height))
 
+    def denoiser(self, seed: int = 1) -> "Denoiser":
+        return Denoiser(self, seed)
+
+    def log_density_input(self, hist_ptr: int, out_ptr: int, npix: int, scale: float):
+        """bf16 16-channel denoiser input from a device RGBA histogram."""
+        self.check(self.L.rf_log_density_input(self.h, C.c_void_p(hist_ptr), C.c_void_p(out_ptr), npix, scale))
+
     def batch(self, frames) -> "Batch":
         """Batched rendering of (flame, width, height, samples) frames."""
         return Batch(self, frames)
@@ -688,3 +715,50 @@ class Batch:
 
     def image_ptr(self, i: int) -> int:
         return self.ctx.L.rf_batch_image_ptr(self.h, i)
+
+
+class Denoiser:
+    """The U-Net denoiser (rf_denoiser_*).  Tensors are passed as device pointers."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        h = _P()
+        ctx.check(ctx.L.rf_denoiser_create(ctx.h, seed, C.byref(h)))
+        self.h = h
+        self.n_params = int(ctx.L.rf_denoiser_num_params(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.rf_denoiser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        self.ctx.check(self.ctx.L.rf_denoiser_get_params(self.h, out.ctypes.data, self.n_params))
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        self.ctx.check(self.ctx.L.rf_denoiser_set_params(self.h, p.ctypes.data, p.size))
+
+    def train_step(self, x_ptr: int, t_ptr: int, N: int, H: int, W: int, lr: float, want_loss=True):
+        loss = C.c_float(0.0)
+        self.ctx.check(self.ctx.L.rf_denoiser_train_step(self.h, C.c_void_p(x_ptr), C.c_void_p(t_ptr), N, H, W,
+                                                         lr, C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def infer(self, x_ptr: int, y_ptr: int, N: int, H: int, W: int):
+        self.ctx.check(self.ctx.L.rf_denoiser_infer(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), N, H, W))
+
+    def grads(self, x_ptr: int, t_ptr: int, N: int, H: int, W: int):
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float(0.0)
+        self.ctx.check(self.ctx.L.rf_denoiser_grads(self.h, C.c_void_p(x_ptr), C.c_void_p(t_ptr), N, H, W,
+                                                    g.ctypes.data, C.byref(loss)))
+        return g, loss.value

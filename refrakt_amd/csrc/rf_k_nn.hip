@@ -1,0 +1,370 @@
+// Denoiser network kernels (gfx950): bf16 MFMA convolutions with fp32
+// accumulation, weight gradients, pooling, loss and Adam.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "rf_nn.hpp"
+
+namespace rfn {
+namespace {
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Round to nearest even; NaN stays NaN.
+__device__ __forceinline__ bf16 f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)0x7fc0u;
+  return (bf16)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// ------------------------------------------------------------ convolution
+// Implicit GEMM: M = pixels, N = output channels, K = 9 * Cin.  A workgroup
+// of 4 waves covers 64 consecutive pixels of one image row (16 per wave) and
+// 16*NT output channels.  Per (r, s, 16-channel step) each wave issues NT
+// v_mfma_f32_16x16x16_bf16 with one A fragment (16 pixels x 16 channels) and
+// NT B fragments (16 channels x 16 outputs).  Operand layout of that
+// instruction (wave64): lane l holds A[l % 16][4 (l / 16) + e] and
+// B[4 (l / 16) + e][l % 16] for e = 0..3, and D[4 (l / 16) + e][l % 16].
+// NHWC activations and [r][s][Cout][Cin] weights make both fragments one
+// 8-byte load per lane.
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3x3(View in, const bf16* __restrict__ w, const float* __restrict__ bias,
+                                                 View out, int H, int W, int Cin, int Cout, int relu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = blockIdx.y / H, y = blockIdx.y % H;
+  const int x0 = blockIdx.x * 64 + wave * 16;
+  const int co0 = blockIdx.z * 16 * NT;
+  const int i = lane & 15, kq = lane >> 4;
+  float4_t acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const int xa = x0 + i;
+  for (int r = 0; r < 3; ++r) {
+    const int yy = y + r - 1;
+    const bool rok = yy >= 0 && yy < H;
+    for (int s = 0; s < 3; ++s) {
+      const int xx = xa + s - 1;
+      const bool ok = rok && xx >= 0 && xx < W;
+      const bf16* ap = in.p + ((size_t)(n * H + (rok ? yy : 0)) * W + (ok ? xx : 0)) * in.cs + in.co + 4 * kq;
+      const bf16* wp = w + ((size_t)(r * 3 + s) * Cout + co0 + i) * Cin + 4 * kq;
+      for (int c0 = 0; c0 < Cin; c0 += 16) {
+        short4_t a = short4_t{0, 0, 0, 0};
+        if (ok) a = *reinterpret_cast<const short4_t*>(ap + c0);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const short4_t b = *reinterpret_cast<const short4_t*>(wp + (size_t)t * 16 * Cin + c0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = co0 + 16 * t + i;
+    const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int px = x0 + 4 * kq + e;
+      if (px < W) {
+        float v = acc[t][e] + bv;
+        if (relu) v = fmaxf(v, 0.0f);
+        out.p[((size_t)(n * H + y) * W + px) * out.cs + out.co + co] = f2bf(v);
+      }
+    }
+  }
+}
+
+// Weight gradient on the VALU: a workgroup owns a 16 (out) x 16 (in) channel
+// block, one thread per (k, c) pair with the 9 taps in registers, and walks
+// image rows; each 64-pixel segment of dy and the three input rows around it
+// are staged in LDS as fp32.  Partial sums go to dw/db with atomics.
+constexpr int WG_SEG = 64;
+
+__global__ __launch_bounds__(256) void k_wgrad(View x, View dy, float* __restrict__ dw, float* __restrict__ db,
+                                               int N, int H, int W, int Cin, int Cout) {
+  __shared__ float s_dy[WG_SEG][16];
+  __shared__ float s_x[3][WG_SEG + 2][17];
+  const int tid = threadIdx.x;
+  const int kl = tid >> 4, cl = tid & 15;
+  const int nkb = Cout / 16;
+  const int kb = blockIdx.y % nkb, cb = blockIdx.y / nkb;
+  const int k = kb * 16 + kl, c = cb * 16 + cl;
+  float acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
+  float accb = 0.0f;
+  const int rows = N * H;
+  for (int row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int n = row / H, y = row % H;
+    for (int xs = 0; xs < W; xs += WG_SEG) {
+      __syncthreads();
+      for (int q = tid; q < WG_SEG * 16; q += 256) {
+        const int px = q >> 4, ch = q & 15;
+        const int xx = xs + px;
+        s_dy[px][ch] = xx < W ? bf2f(dy.p[((size_t)(n * H + y) * W + xx) * dy.cs + dy.co + kb * 16 + ch]) : 0.0f;
+      }
+      for (int q = tid; q < 3 * (WG_SEG + 2) * 16; q += 256) {
+        const int ch = q & 15, rest = q >> 4;
+        const int px = rest % (WG_SEG + 2), r = rest / (WG_SEG + 2);
+        const int yy = y + r - 1, xx = xs + px - 1;
+        float v = 0.0f;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+          v = bf2f(x.p[((size_t)(n * H + yy) * W + xx) * x.cs + x.co + cb * 16 + ch]);
+        s_x[r][px][ch] = v;
+      }
+      __syncthreads();
+      float w0[3], w1[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) { w0[r] = s_x[r][0][cl]; w1[r] = s_x[r][1][cl]; }
+      for (int px = 0; px < WG_SEG; ++px) {
+        const float d = s_dy[px][kl];
+        accb += d;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const float w2 = s_x[r][px + 2][cl];
+          acc[r * 3 + 0] = fmaf(d, w0[r], acc[r * 3 + 0]);
+          acc[r * 3 + 1] = fmaf(d, w1[r], acc[r * 3 + 1]);
+          acc[r * 3 + 2] = fmaf(d, w2, acc[r * 3 + 2]);
+          w0[r] = w1[r];
+          w1[r] = w2;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) atomicAdd(dw + ((size_t)q * Cout + k) * Cin + c, acc[q]);
+  if (db && cb == 0 && cl == 0) atomicAdd(db + k, accb);
+}
+
+__global__ void k_flip(const bf16* __restrict__ w, bf16* __restrict__ wt, int Cin, int Cout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = 9 * Cin * Cout;
+  if (i >= n) return;
+  // wt index: [r][s][c][k] with k fastest
+  const int k = i % Cout, c = (i / Cout) % Cin, rs = i / (Cout * Cin);
+  const int r = rs / 3, s = rs % 3;
+  wt[i] = w[((size_t)((2 - r) * 3 + (2 - s)) * Cout + k) * Cin + c];
+}
+
+__global__ void k_relu_mask(View dy, View y, int npix, int C) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)npix * C) return;
+  const size_t p = i / C, c = i % C;
+  bf16* d = dy.p + p * dy.cs + dy.co + c;
+  if (!(bf2f(y.p[p * y.cs + y.co + c]) > 0.0f)) *d = 0;
+}
+
+__global__ void k_avgpool2(View in, View out, int N, int H, int W, int C) {
+  const int Ho = H / 2, Wo = W / 2;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N * Ho * Wo * C) return;
+  const int c = (int)(i % C);
+  const size_t po = i / C;
+  const int xo = (int)(po % Wo), yo = (int)((po / Wo) % Ho), n = (int)(po / ((size_t)Wo * Ho));
+  float s = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) s += bf2f(in.p[((size_t)(n * H + 2 * yo + a) * W + 2 * xo + b) * in.cs + in.co + c]);
+  out.p[po * out.cs + out.co + c] = f2bf(0.25f * s);
+}
+
+__global__ void k_avgpool2_bwd(View dout, View din, int N, int H, int W, int C, int acc) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N * H * W * C) return;
+  const int c = (int)(i % C);
+  const size_t p = i / C;
+  const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
+  const float g = 0.25f * bf2f(dout.p[((size_t)(n * (H / 2) + y / 2) * (W / 2) + x / 2) * dout.cs + dout.co + c]);
+  bf16* d = din.p + p * din.cs + din.co + c;
+  *d = f2bf(acc ? bf2f(*d) + g : g);
+}
+
+__global__ void k_upsample2(View in, View out, int N, int H, int W, int C) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int Ho = 2 * H, Wo = 2 * W;
+  if (i >= (size_t)N * Ho * Wo * C) return;
+  const int c = (int)(i % C);
+  const size_t po = i / C;
+  const int xo = (int)(po % Wo), yo = (int)((po / Wo) % Ho), n = (int)(po / ((size_t)Wo * Ho));
+  out.p[po * out.cs + out.co + c] = in.p[((size_t)(n * H + yo / 2) * W + xo / 2) * in.cs + in.co + c];
+}
+
+__global__ void k_upsample2_bwd(View dout, View din, int N, int H, int W, int C) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N * H * W * C) return;
+  const int c = (int)(i % C);
+  const size_t p = i / C;
+  const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
+  float s = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+      s += bf2f(dout.p[((size_t)(n * 2 * H + 2 * y + a) * (2 * W) + 2 * x + b) * dout.cs + dout.co + c]);
+  din.p[p * din.cs + din.co + c] = f2bf(s);
+}
+
+__global__ void k_residual_loss(View x, View o, const bf16* __restrict__ t, float* __restrict__ y_out,
+                                float* __restrict__ loss_sum, bf16* __restrict__ dO, int npix, float inv_count) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (p < npix) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const float y = bf2f(x.p[(size_t)p * x.cs + x.co + c]) + bf2f(o.p[(size_t)p * o.cs + o.co + c]);
+      if (y_out) y_out[(size_t)p * 4 + c] = y;
+      if (t) {
+        const float d = y - bf2f(t[(size_t)p * 16 + c]);
+        l += d * d;
+        dO[(size_t)p * 16 + c] = f2bf(2.0f * d * inv_count);
+      }
+    }
+    if (t) {
+#pragma unroll
+      for (int c = 4; c < 16; ++c) dO[(size_t)p * 16 + c] = 0;
+    }
+  }
+  if (t) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) l += __shfl_xor(l, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss_sum, l);
+  }
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, bf16* __restrict__ pb, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  // bias-corrected step: c1 = 1 - b1^t, c2 = 1 - b2^t
+  const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  p[i] = pi;
+  if (pb) pb[i] = f2bf(pi);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ in, bf16* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = f2bf(in[i]);
+}
+
+__global__ void k_log_density(const float* __restrict__ hist, bf16* __restrict__ out, int npix, float scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix * 16) return;
+  const int p = i >> 4, c = i & 15;
+  out[i] = c < 4 ? f2bf(log10f(1.0f + hist[(size_t)p * 4 + c] * scale)) : (bf16)0;
+}
+
+__global__ void k_bf16_to_f32_4(const bf16* __restrict__ in, int cs, float* __restrict__ out, int npix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix * 4) return;
+  out[i] = bf2f(in[(size_t)(i >> 2) * cs + (i & 3)]);
+}
+
+__global__ void k_zero(float* __restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace
+
+hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, int H, int W, int Cin, int Cout,
+                   int relu, hipStream_t s) {
+  if (Cin % 16 || Cout % 16 || in.cs % 4 || in.co % 4 || out.cs < out.co + Cout || in.cs < in.co + Cin)
+    return hipErrorInvalidValue;
+  const int nt = (Cout % 64 == 0) ? 4 : (Cout % 32 == 0) ? 2 : 1;
+  const dim3 grid((W + 63) / 64, N * H, Cout / (16 * nt));
+  switch (nt) {
+    case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
+    case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
+    default: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t conv3x3_wgrad(View x, View dy, float* dw, float* db, int N, int H, int W, int Cin, int Cout,
+                         hipStream_t s) {
+  if (Cin % 16 || Cout % 16) return hipErrorInvalidValue;
+  const int rows = N * H;
+  const dim3 grid((unsigned)std::min(rows, 256), (unsigned)((Cout / 16) * (Cin / 16)));
+  hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, s, x, dy, dw, db, N, H, W, Cin, Cout);
+  return hipGetLastError();
+}
+
+hipError_t flip_weights(const bf16* w, bf16* wt, int Cin, int Cout, hipStream_t s) {
+  hipLaunchKernelGGL(k_flip, grid1((size_t)9 * Cin * Cout), dim3(256), 0, s, w, wt, Cin, Cout);
+  return hipGetLastError();
+}
+
+hipError_t relu_mask(View dy, View y, int npix, int C, hipStream_t s) {
+  hipLaunchKernelGGL(k_relu_mask, grid1((size_t)npix * C), dim3(256), 0, s, dy, y, npix, C);
+  return hipGetLastError();
+}
+
+hipError_t avgpool2(View in, View out, int N, int H, int W, int C, hipStream_t s) {
+  hipLaunchKernelGGL(k_avgpool2, grid1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s, in, out, N, H, W, C);
+  return hipGetLastError();
+}
+
+hipError_t avgpool2_bwd(View dout, View din, int N, int H, int W, int C, int acc, hipStream_t s) {
+  hipLaunchKernelGGL(k_avgpool2_bwd, grid1((size_t)N * H * W * C), dim3(256), 0, s, dout, din, N, H, W, C, acc);
+  return hipGetLastError();
+}
+
+hipError_t upsample2(View in, View out, int N, int H, int W, int C, hipStream_t s) {
+  hipLaunchKernelGGL(k_upsample2, grid1((size_t)N * 4 * H * W * C), dim3(256), 0, s, in, out, N, H, W, C);
+  return hipGetLastError();
+}
+
+hipError_t upsample2_bwd(View dout, View din, int N, int H, int W, int C, hipStream_t s) {
+  hipLaunchKernelGGL(k_upsample2_bwd, grid1((size_t)N * H * W * C), dim3(256), 0, s, dout, din, N, H, W, C);
+  return hipGetLastError();
+}
+
+hipError_t residual_loss(View x, View o, const bf16* t, float* y_out, float* loss_sum, bf16* dO, int npix,
+                         float inv_count, hipStream_t s) {
+  hipLaunchKernelGGL(k_residual_loss, grid1((size_t)npix), dim3(256), 0, s, x, o, t, y_out, loss_sum, dO, npix,
+                     inv_count);
+  return hipGetLastError();
+}
+
+hipError_t adam(float* p, const float* g, float* m, float* v, bf16* pb, int n, float lr, float b1, float b2,
+                float eps, float c1, float c2, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, grid1((size_t)n), dim3(256), 0, s, p, g, m, v, pb, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t to_bf16(const float* in, bf16* out, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_bf16, grid1((size_t)n), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t log_density_bf16(const float* hist, bf16* out, int npix, float scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_log_density, grid1((size_t)npix * 16), dim3(256), 0, s, hist, out, npix, scale);
+  return hipGetLastError();
+}
+
+hipError_t bf16_to_f32_4(const bf16* in, int cs, float* out, int npix, hipStream_t s) {
+  hipLaunchKernelGGL(k_bf16_to_f32_4, grid1((size_t)npix * 4), dim3(256), 0, s, in, cs, out, npix);
+  return hipGetLastError();
+}
+
+hipError_t zero_f32(float* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero, grid1(n), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace rfn
