@@ -67,11 +67,14 @@ extern "C" {
 
 /* Histogram accumulation strategies of the fast engine (the compat engine
  * uses QUAD for AUTO and LDS). */
-#define RF_HIST_AUTO 0     /* LDS first; QUAD from then on if under 2 % of the samples hit the LDS tables */
+#define RF_HIST_AUTO 0     /* LDS first; after one draw SORT if under 90 % of the samples re-hit the LDS
+                              tables (QUAD if SORT is not allowed and under 2 %) */
 #define RF_HIST_QUAD 1     /* 4 lanes per RGBA bin: one 16-byte atomic request per sample */
 #define RF_HIST_WAVEAGG 2  /* merge same-bin hits inside a wave before the atomics */
 #define RF_HIST_NAIVE 3    /* 4 separate float atomics per sample (baseline) */
 #define RF_HIST_LDS 4      /* per-workgroup LDS table of hot bins, flushed once per launch; misses go QUAD */
+#define RF_HIST_SORT 5     /* records sorted into LDS-sized tiles, summed in LDS: no per-sample atomics;
+                              needs unit opacities (else QUAD) */
 
 typedef struct rf_ctx rf_ctx;
 typedef struct rf_flame rf_flame;
@@ -358,6 +361,9 @@ int rf_fast_perms_download(rf_ctx* ctx, uint16_t* host);
  * RF_HIST_AUTO) and, if a draw has used RF_HIST_LDS, the fraction of its binned
  * samples whose bin was already in their workgroup's LDS table (else -1). */
 int rf_flame_hist_mode(const rf_flame* flame, double* lds_hit_fraction);
+/* Fraction of the binned samples of the flame's last RF_HIST_SORT draw that
+ * overflowed the bucket/tile capacities (and were added with atomics), or -1. */
+double rf_flame_sort_overflow(const rf_flame* flame);
 /* Screen-space affine used by draw_to_bins (src/flame.cpp:289-296). */
 int rf_screen_affine(const rf_flame_desc* desc, uint32_t width, uint32_t height, float out[6]);
 

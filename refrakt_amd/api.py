@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "librefrakt_amd.so")
 RF_OK = 0
 RF_ENGINE_COMPAT = 0
 RF_ENGINE_FAST = 1
-RF_HIST_AUTO, RF_HIST_QUAD, RF_HIST_WAVEAGG, RF_HIST_NAIVE, RF_HIST_LDS = 0, 1, 2, 3, 4
+RF_HIST_AUTO, RF_HIST_QUAD, RF_HIST_WAVEAGG, RF_HIST_NAIVE, RF_HIST_LDS, RF_HIST_SORT = 0, 1, 2, 3, 4, 5
 MAX_XFORMS = 32
 MAX_VARS = 16
 MAX_PARAMS = 8
@@ -129,6 +129,7 @@ _SIGS = {
     "rf_shuffle_download": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "rf_sample_points_download": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "rf_flame_hist_mode": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "rf_flame_sort_overflow": (C.c_double, [_P]),
     "rf_batch_create": (C.c_int, [_P, C.POINTER(FrameDescC), C.c_uint32, C.POINTER(_P)]),
     "rf_batch_destroy": (None, [_P]),
     "rf_batch_set_config": (C.c_int, [_P, C.POINTER(BatchConfig)]),
@@ -580,6 +581,10 @@ class FlameHandle:
         if m < 0:
             self.ctx.check(m)
         return m, frac.value
+
+    def sort_overflow(self) -> float:
+        """Overflow fraction of the last sort-mode draw (-1 if none)."""
+        return float(self.ctx.L.rf_flame_sort_overflow(self.h))
 
     def param_block(self) -> np.ndarray:
         return param_block(self.flame)

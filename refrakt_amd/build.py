@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librefrakt_amd.so")
 SOURCES = ["rf_k_compat.hip", "rf_k_fast.hip", "rf_k_misc.hip", "rf_host.cpp", "rf_flam3.cpp", "rf_output.cpp",
-           "rf_k_batch.hip", "rf_batch.cpp", "rf_k_nn.hip", "rf_denoiser.cpp"]
+           "rf_k_batch.hip", "rf_batch.cpp", "rf_k_nn.hip", "rf_denoiser.cpp", "rf_k_sort.hip"]
 ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

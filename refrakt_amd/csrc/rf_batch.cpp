@@ -32,6 +32,7 @@ struct rf_batch {
   hipGraphExec_t exec = nullptr;
   double clock_khz = 0.0;
   uint32_t launch = 0;             // RNG stream key of this batch (same for every shape)
+  bool lds_ok = true;              // every frame allows the fixed-point LDS path
   bool ran = false;
 
   ~rf_batch() {
@@ -107,11 +108,12 @@ int capture(rf_batch* b) {
   if (b->exec) { HIPCHK(ctx, hipGraphExecDestroy(b->exec)); b->exec = nullptr; }
   if (b->graph) { HIPCHK(ctx, hipGraphDestroy(b->graph)); b->graph = nullptr; }
   // Size the persistent grid outside the capture (occupancy queries).
-  HIPCHK(ctx, rfk::batch_prepare(c.block_waves, c.hist_mode));
+  HIPCHK(ctx, rfk::batch_prepare(c.block_waves, (c.hist_mode == RF_HIST_LDS && !b->lds_ok) ? RF_HIST_QUAD : c.hist_mode));
   HIPCHK(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipError_t e = rfk::launch_batch_init(a, (float4*)b->hist.p, b->total_bins, (const uint32_t*)b->items.p,
                                         (uint32_t*)b->tiles_done.p, (unsigned long long*)b->t0.p, b->stream);
-  if (e == hipSuccess) e = rfk::launch_batch(a, c.block_waves, c.hist_mode, 0, b->stream);
+  const int hm = (c.hist_mode == RF_HIST_LDS && !b->lds_ok) ? RF_HIST_QUAD : c.hist_mode;
+  if (e == hipSuccess) e = rfk::launch_batch(a, c.block_waves, hm, 0, b->stream);
   for (uint32_t f = 0; f < b->n && e == hipSuccess; ++f) {
     const rf_flame_desc& d = b->desc[f];
     rfk::DEArgs g{};
@@ -183,6 +185,7 @@ int rf_batch_create(rf_ctx* ctx, const rf_frame_desc* frames, uint32_t n, rf_bat
       return rfh::set_err(ctx, RF_ERR_ARG, "batch: frame " + std::to_string(f) + ": " + err);
     }
     b->desc.push_back(*fr.flame);
+    if (!rfh::fixed_point_ok(*fr.flame)) b->lds_ok = false;
     b->W.push_back(fr.width);
     b->H.push_back(fr.height);
     b->samples.push_back(fr.samples);

@@ -237,6 +237,16 @@ int rfh::set_err(rf_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
+bool rfh::fixed_point_ok(const rf_flame_desc& d) {
+  for (int i = 0; i < RF_PALETTE_SIZE; ++i)
+    for (int c = 0; c < 3; ++c)
+      if (!(d.palette[i][c] >= 0.0f && d.palette[i][c] <= 1.0f)) return false;
+  for (int i = 0; i < d.n_xforms; ++i)
+    if (!(d.xforms[i].opacity >= 0.0f && d.xforms[i].opacity <= 1.0f)) return false;
+  if (d.has_final && !(d.final_xform.opacity >= 0.0f && d.final_xform.opacity <= 1.0f)) return false;
+  return true;
+}
+
 std::vector<uint16_t> rfh::perm_rows(uint64_t seed, int waves) {
   const uint32_t N = 64u * (uint32_t)waves;
   std::vector<uint16_t> tab((size_t)rfk::NPERM * N);
@@ -346,7 +356,7 @@ static int flame_upload(rf_flame* f, float tss_width) {
   HIPCHK(ctx, hipMemcpyAsync(f->palette.p, f->desc.palette, 256 * 16, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   f->needs_update = false;
-  f->auto_mode = RF_HIST_LDS;
+  f->auto_mode = fixed_point_ok(f->desc) ? RF_HIST_LDS : RF_HIST_QUAD;
   return RF_OK;
 }
 
@@ -428,12 +438,129 @@ static int do_warmup(rf_ctx* ctx, rf_flame* f, uint32_t num_passes, float tss, c
   return RF_OK;
 }
 
+// RF_HIST_SORT needs unit sample weights (records carry no weight), palette
+// values in [0, 1] (fixed-point tile sums) and at most 2^27 bins.
+static bool sort_eligible(const rf_flame* f, uint32_t W, uint32_t H) {
+  if ((uint64_t)W * H > ((uint64_t)rfk::SORT_MAX_BUCKETS << rfk::SORT_COARSE_SHIFT)) return false;
+  if (!fixed_point_ok(f->desc)) return false;
+  for (int i = 0; i < f->desc.n_xforms; ++i)
+    if (f->desc.xforms[i].opacity != 1.0f) return false;
+  if (f->desc.has_final && f->desc.final_xform.opacity != 1.0f) return false;
+  return true;
+}
+
+// Capacities of the next sort draw from the last one's distribution (20 %
+// headroom), or 3x the mean where there is no history; uploads offsets and
+// capacities and grows the record buffers.  meta layout: boff, bcap, toff, tcap.
+static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, rfk::SortArgs& a,
+                        uint32_t& max_bcap, std::vector<uint32_t>& meta) {
+  const uint32_t nb = (uint32_t)((n_bins + (1u << rfk::SORT_COARSE_SHIFT) - 1) >> rfk::SORT_COARSE_SHIFT);
+  const uint32_t nt = nb * rfk::SORT_TILES_PER_BUCKET;
+  const uint32_t nt_used = (uint32_t)((n_bins + (1u << rfk::SORT_TILE_SHIFT) - 1) >> rfk::SORT_TILE_SHIFT);
+  if (f->sort_bins != n_bins || f->sort_bfrac.size() != nb) {
+    f->sort_bins = n_bins;
+    f->sort_bfrac.assign(nb, 3.0 / nb);
+    f->sort_tfrac.assign(nt, 0.0);
+    for (uint32_t t = 0; t < nt_used; ++t) f->sort_tfrac[t] = 3.0 / nt_used;
+  }
+  meta.assign(2 * (size_t)nb + 2 * (size_t)nt, 0u);
+  uint32_t* boff = meta.data();
+  uint32_t* bcap = boff + nb;
+  uint32_t* toff = bcap + nb;
+  uint32_t* tcap = toff + nt;
+  uint64_t cb = 0, ct = 0;
+  max_bcap = 0;
+  for (uint32_t b = 0; b < nb; ++b) {
+    const uint64_t c = (uint64_t)std::ceil((double)S * f->sort_bfrac[b] * 1.2) + 2048;
+    boff[b] = (uint32_t)cb;
+    bcap[b] = (uint32_t)std::min<uint64_t>(c, 0x7fffffffull);
+    cb += bcap[b];
+    max_bcap = std::max(max_bcap, bcap[b]);
+  }
+  for (uint32_t t = 0; t < nt; ++t) {
+    uint64_t c = t < nt_used ? (uint64_t)std::ceil((double)S * f->sort_tfrac[t] * 1.2) + 512 : 0;
+    c = (c + 3) & ~3ull;  // every tile's records start 16-byte aligned
+    toff[t] = (uint32_t)ct;
+    tcap[t] = (uint32_t)std::min<uint64_t>(c, 0x7ffffffcull);
+    ct += tcap[t];
+  }
+  if (cb >= 0xffffffffull || ct >= 0xffffffffull)
+    return set_err(ctx, RF_ERR_LIMIT, "sort histogram: more than 2^32 records per draw; use fewer passes per call");
+  // work lists: rebucketing chunks per bucket, summing segments per used tile
+  const size_t meta_fixed = meta.size();
+  std::vector<uint32_t> jobs;
+  for (uint32_t b = 0; b < nb; ++b)
+    for (uint32_t k = 0; k * (uint64_t)rfk::SORT_CHUNK < bcap[b]; ++k) {
+      if (k > 0xffffu) return set_err(ctx, RF_ERR_LIMIT, "sort histogram: bucket too large");
+      jobs.push_back((b << 16) | k);
+    }
+  const uint32_t n_chunks = (uint32_t)jobs.size();
+  for (uint32_t t = 0; t < nt_used; ++t)
+    for (uint32_t k = 0; k * (uint64_t)rfk::SORT_SEG < tcap[t]; ++k) {
+      if (k > 0xffffu) return set_err(ctx, RF_ERR_LIMIT, "sort histogram: tile too large");
+      jobs.push_back((t << 16) | k);
+    }
+  const uint32_t n_segs = (uint32_t)jobs.size() - n_chunks;
+  meta.insert(meta.end(), jobs.begin(), jobs.end());
+  if (ctx->sort_cbuf.bytes < cb * 4) HIPCHK(ctx, ctx->sort_cbuf.alloc(cb * 4));
+  if (ctx->sort_tbuf.bytes < ct * 4) HIPCHK(ctx, ctx->sort_tbuf.alloc(ct * 4));
+  if (ctx->sort_bcur.bytes < (size_t)nb * 4) HIPCHK(ctx, ctx->sort_bcur.alloc((size_t)rfk::SORT_MAX_BUCKETS * 4));
+  if (ctx->sort_tcur.bytes < (size_t)nt * 4)
+    HIPCHK(ctx, ctx->sort_tcur.alloc((size_t)rfk::SORT_MAX_BUCKETS * rfk::SORT_TILES_PER_BUCKET * 4));
+  if (ctx->sort_meta.bytes < meta.size() * 4) HIPCHK(ctx, ctx->sort_meta.alloc(meta.size() * 4 * 2));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->sort_meta.p, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  const uint32_t* dm = (const uint32_t*)ctx->sort_meta.p;
+  a.cbuf = (uint32_t*)ctx->sort_cbuf.p;
+  a.tbuf = (uint32_t*)ctx->sort_tbuf.p;
+  a.bcur = (uint32_t*)ctx->sort_bcur.p;
+  a.tcur = (uint32_t*)ctx->sort_tcur.p;
+  a.boff = dm;
+  a.bcap = dm + nb;
+  a.toff = dm + 2 * (size_t)nb;
+  a.tcap = dm + 2 * (size_t)nb + nt;
+  a.chunks = dm + meta_fixed;
+  a.segs = dm + meta_fixed + n_chunks;
+  a.n_buckets = nb;
+  a.n_tiles = nt;
+  a.n_chunks = n_chunks;
+  a.n_segs = n_segs;
+  (void)max_bcap;
+  return RF_OK;
+}
+
+// After a sort draw: record where the samples went and how many overflowed.
+static int sort_finish(rf_ctx* ctx, rf_flame* f, const rfk::SortArgs& a, uint64_t S, uint64_t drawn,
+                       const std::vector<uint32_t>& meta) {
+  std::vector<uint32_t> bc(a.n_buckets), tc(a.n_tiles);
+  HIPCHK(ctx, hipMemcpy(bc.data(), a.bcur, bc.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(tc.data(), a.tcur, tc.size() * 4, hipMemcpyDeviceToHost));
+  const uint32_t* bcap = meta.data() + a.n_buckets;
+  const uint32_t* tcap = meta.data() + 2 * (size_t)a.n_buckets + a.n_tiles;
+  uint64_t ovf = 0;
+  const double inv = S ? 1.0 / (double)S : 0.0;
+  for (uint32_t b = 0; b < a.n_buckets; ++b) {
+    if (bc[b] > bcap[b]) ovf += bc[b] - bcap[b];
+    f->sort_bfrac[b] = bc[b] * inv;
+  }
+  for (uint32_t t = 0; t < a.n_tiles; ++t) {
+    if (tc[t] > tcap[t]) ovf += tc[t] - tcap[t];
+    // a bucket's overflow never reaches its tiles: keep their share at least the mean
+    f->sort_tfrac[t] = tc[t] * inv;
+  }
+  f->last_sort_overflow = drawn ? (double)ovf / (double)drawn : 0.0;
+  return RF_OK;
+}
+
 // RF_HIST_AUTO keeps the LDS bin cache while at least this fraction of the
 // binned samples finds its bin in a workgroup's table.  Measured on MI355X the
 // cache already breaks even with the quad path at a 4 % hit fraction (a
 // 7680x4320 view of an 8-xform flame), so it is only dropped when the table
 // is nearly useless.
 static constexpr double kAutoLdsMinHits = 0.02;
+// ... and prefers the sort path below this fraction when the flame allows it:
+// only a table that catches nearly every sample (a few hundred hot bins, where
+// the sort path's tiles would be hot too) beats it.
+static constexpr double kAutoSortMaxHits = 0.90;
 
 static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, const uint32_t* rows_in,
                    uint64_t* drawn) {
@@ -445,6 +572,10 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
   screen_affine(f->desc, h->w, h->h, ss);
   HIPCHK(ctx, hipMemsetAsync(ctx->counter.p, 0, 16, ctx->stream));
   const int hm = ctx->sp.hist_mode == RF_HIST_AUTO ? f->auto_mode : ctx->sp.hist_mode;
+  bool sort_used = false;
+  rfk::SortArgs sort_args{};
+  std::vector<uint32_t> sort_meta;
+  uint64_t sort_S = 0;
   if (!ctx->ev[2]) { HIPCHK(ctx, hipEventCreate(&ctx->ev[2])); HIPCHK(ctx, hipEventCreate(&ctx->ev[3])); }
   if (ctx->sp.engine == RF_ENGINE_COMPAT) {
     int rc = ensure_compat(ctx);
@@ -477,16 +608,44 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     a.bins = (float*)h->ptr;
     a.W = h->w; a.H = h->h;
     memcpy(a.ss, ss, sizeof(ss));
+    int hm_run = hm;
+    if (hm_run == RF_HIST_SORT && !sort_eligible(f, h->w, h->h)) hm_run = RF_HIST_QUAD;
+    if (hm_run == RF_HIST_LDS && !fixed_point_ok(f->desc)) hm_run = RF_HIST_QUAD;
+    std::vector<uint32_t> meta;
+    uint32_t max_bcap = 0;
+    const uint64_t S = P * (uint64_t)num_iter;
+    if (hm_run == RF_HIST_SORT && num_iter) {
+      const int rc2 = sort_prepare(ctx, f, h->w * h->h, S, a.sort, max_bcap, meta);
+      if (rc2) return rc2;
+    }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    if (num_iter) HIPCHK(ctx, rfk::launch_fast(a, P, fast_waves(ctx), hm, ctx->stream));
+    if (num_iter) {
+      if (hm_run == RF_HIST_SORT) HIPCHK(ctx, rfk::launch_sort_reset(a.sort, ctx->stream));
+      HIPCHK(ctx, rfk::launch_fast(a, P, fast_waves(ctx), hm_run, ctx->stream));
+      if (hm_run == RF_HIST_SORT) {
+        HIPCHK(ctx, rfk::launch_sort_rebucket(a.sort, a.bins, a.palette, ctx->stream));
+        HIPCHK(ctx, rfk::launch_sort_tiles(a.sort, a.bins, h->w * h->h, a.palette, ctx->stream));
+      }
+    }
+    sort_used = hm_run == RF_HIST_SORT && num_iter;
+    sort_args = a.sort;
+    sort_meta.swap(meta);
+    sort_S = S;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   uint64_t cnt[2] = {0, 0};
   HIPCHK(ctx, hipMemcpyAsync(cnt, ctx->counter.p, 16, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (sort_used) {
+    const int rc2 = sort_finish(ctx, f, sort_args, sort_S, cnt[0], sort_meta);
+    if (rc2) return rc2;
+  }
   if (ctx->sp.engine == RF_ENGINE_FAST && hm == RF_HIST_LDS && cnt[0]) {
     f->last_lds_hit_fraction = (double)cnt[1] / (double)cnt[0];
-    if (ctx->sp.hist_mode == RF_HIST_AUTO && f->last_lds_hit_fraction < kAutoLdsMinHits) f->auto_mode = RF_HIST_QUAD;
+    if (ctx->sp.hist_mode == RF_HIST_AUTO && f->last_lds_hit_fraction < kAutoSortMaxHits) {
+      if (sort_eligible(f, h->w, h->h)) f->auto_mode = RF_HIST_SORT;
+      else if (f->last_lds_hit_fraction < kAutoLdsMinHits) f->auto_mode = RF_HIST_QUAD;
+    }
   }
   float ms = 0.0f;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
@@ -552,7 +711,7 @@ int rf_create(const rf_sim_params* p, rf_ctx** out) {
     delete ctx;
     return RF_ERR_ARG;
   }
-  if (p->hist_mode < RF_HIST_AUTO || p->hist_mode > RF_HIST_LDS) {
+  if (p->hist_mode < RF_HIST_AUTO || p->hist_mode > RF_HIST_SORT) {
     g_create_err = "unknown hist mode";
     delete ctx;
     return RF_ERR_ARG;
@@ -601,7 +760,7 @@ int rf_set_sim_parameters(rf_ctx* ctx, uint64_t P, uint32_t T, uint32_t R) {
 int rf_set_engine(rf_ctx* ctx, int32_t engine, int32_t hist_mode) {
   if (!ctx) return RF_ERR_ARG;
   if (engine != RF_ENGINE_COMPAT && engine != RF_ENGINE_FAST) return set_err(ctx, RF_ERR_ARG, "unknown engine");
-  if (hist_mode < RF_HIST_AUTO || hist_mode > RF_HIST_LDS) return set_err(ctx, RF_ERR_ARG, "unknown hist mode");
+  if (hist_mode < RF_HIST_AUTO || hist_mode > RF_HIST_SORT) return set_err(ctx, RF_ERR_ARG, "unknown hist mode");
   if (engine != ctx->sp.engine) ctx->generation++;  // particle layouts differ between engines
   ctx->sp.engine = engine;
   ctx->sp.hist_mode = hist_mode;
@@ -693,6 +852,8 @@ int rf_rgba8_device(rf_ctx* ctx, const void* in, void* out, uint32_t w, uint32_t
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return RF_OK;
 }
+
+double rf_flame_sort_overflow(const rf_flame* f) { return f ? f->last_sort_overflow : -1.0; }
 
 int rf_flame_hist_mode(const rf_flame* f, double* frac) {
   if (!f) return RF_ERR_ARG;

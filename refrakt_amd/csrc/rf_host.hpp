@@ -49,6 +49,9 @@ std::vector<float> reference_param_block(const rf_flame_desc& d);
 void screen_affine(const rf_flame_desc& d, uint32_t w, uint32_t h, float out[6]);
 // The fast engine's 16 LDS permutation rows for 64*waves lanes.
 std::vector<uint16_t> perm_rows(uint64_t seed, int waves);
+// True when every palette channel and every opacity lies in [0, 1]: the LDS
+// histogram paths accumulate in fixed point and need that range.
+bool fixed_point_ok(const rf_flame_desc& d);
 // Records msg as the context's last error and returns code.
 int set_err(rf_ctx* ctx, int code, const std::string& msg);
 
@@ -77,6 +80,8 @@ struct rf_ctx {
   rfh::DevBuf perms;
   int perm_waves = 0;
   rfh::DevBuf counter;         // u64 [drawn, samples added in the LDS bin cache]
+  // RF_HIST_SORT buffers: bucket and tile records, cursors, offsets, capacities
+  rfh::DevBuf sort_cbuf, sort_tbuf, sort_bcur, sort_tcur, sort_meta;
   rfh::DevBuf probe;           // atomic probe scratch
   rf_hist* render_hist = nullptr;
   rfh::DevBuf render_img;
@@ -102,6 +107,12 @@ struct rf_flame {
   // kAutoLdsMinHits of its samples in LDS.
   int auto_mode = RF_HIST_LDS;
   double last_lds_hit_fraction = -1.0;
+  // RF_HIST_SORT: fraction of the samples that went to each bucket and tile
+  // in the last draw (sizes the next draw's capacities), and the fraction of
+  // binned samples that overflowed them.
+  uint32_t sort_bins = 0;
+  std::vector<double> sort_bfrac, sort_tfrac;
+  double last_sort_overflow = -1.0;
 };
 
 struct rf_hist {

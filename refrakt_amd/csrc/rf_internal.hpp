@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "rf_device.hpp"
+#include "rf_sort.hpp"
 
 namespace rfk {
 
@@ -42,6 +43,7 @@ struct FastArgs {
   uint32_t first_run, do_draw;
   uint32_t W, H;
   float ss[6];
+  SortArgs sort;                 // RF_HIST_SORT only
 };
 
 struct DEArgs {

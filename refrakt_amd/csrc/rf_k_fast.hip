@@ -27,6 +27,77 @@ using namespace detail;
 // HM 1/2/3 bin straight to the histogram and launch one workgroup per batch;
 // HM 4 (LDS bin cache) launches one workgroup per resident slot so that each
 // workgroup's LDS table spans many batches before it is flushed.
+// HIST_SORT staging: the samples of up to SORT_STAGE_ITERS iterations wait in
+// LDS (one fixed slot per lane and iteration, no append atomics).  A flush
+// ranks them by bucket (counting sort), reserves one run per non-empty bucket
+// with one global atomic, places the records in bucket order in LDS and
+// writes them out so that neighbouring lanes store neighbouring addresses.
+template <int N, bool ON>
+struct SortStage {
+  uint32_t bin[SORT_STAGE_ITERS * N];
+  uint32_t rec[SORT_STAGE_ITERS * N];   // bucket-ordered records
+  uint16_t rank[SORT_STAGE_ITERS * N];
+  uint8_t pid[SORT_STAGE_ITERS * N];
+  uint8_t bkt[SORT_STAGE_ITERS * N];    // bucket of rec[j]
+  uint32_t cnt[SORT_MAX_BUCKETS];
+  uint32_t off[SORT_MAX_BUCKETS];
+  uint32_t gb[SORT_MAX_BUCKETS];
+  uint32_t total;
+};
+template <int N>
+struct SortStage<N, false> {};
+
+template <int N>
+__device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_slots, const SortArgs& a,
+                                           float* __restrict__ bins, const float4* __restrict__ pal) {
+  static_assert(SORT_MAX_BUCKETS <= 4 * WAVE, "scan width");
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t b = tid; b < a.n_buckets; b += N) st.cnt[b] = 0u;
+  lds_barrier();
+  for (uint32_t i = tid; i < n_slots; i += N) {
+    const uint32_t bin = st.bin[i];
+    if (bin != SORT_NONE) st.rank[i] = (uint16_t)atomicAdd(&st.cnt[bin >> SORT_COARSE_SHIFT], 1u);
+  }
+  lds_barrier();
+  wave0_exclusive_scan<4>(st.cnt, st.off, a.n_buckets, &st.total);
+  for (uint32_t b = tid; b < a.n_buckets; b += N) {
+    const uint32_t c = st.cnt[b];
+    st.gb[b] = c ? atomicAdd(a.bcur + b, c) : 0u;
+  }
+  lds_barrier();
+  for (uint32_t i = tid; i < n_slots; i += N) {
+    const uint32_t bin = st.bin[i];
+    if (bin != SORT_NONE) {
+      const uint32_t b = bin >> SORT_COARSE_SHIFT;
+      const uint32_t j = st.off[b] + st.rank[i];
+      st.rec[j] = ((bin & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | st.pid[i];
+      st.bkt[j] = (uint8_t)b;
+    }
+  }
+  lds_barrier();
+  const uint32_t total = st.total;
+  const uint32_t rounds = (total + N - 1) / N;  // uniform
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t j = r * N + tid;
+    bool ovf = false;
+    uint32_t bin = 0, pidx = 0;
+    if (j < total) {
+      const uint32_t b = st.bkt[j];
+      const uint32_t rec = st.rec[j];
+      const uint32_t dst = st.gb[b] + (j - st.off[b]);
+      if (dst < a.bcap[b]) {
+        a.cbuf[a.boff[b] + dst] = rec;
+      } else {  // over the bucket's capacity: straight to the histogram
+        ovf = true;
+        bin = (b << SORT_COARSE_SHIFT) | (rec >> 8);
+        pidx = rec & 255u;
+      }
+    }
+    bin_quad(bins, bin, ovf, pidx, 1.0f, pal);
+  }
+  lds_barrier();
+}
+
 template <int WAVES, int HM, int LOGK>
 __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __restrict__ blocks) {
   constexpr int N = 64 * WAVES;
@@ -39,10 +110,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __
   // behind the iteration's atomics in the in-order vmcnt queue and stall.
   __shared__ uint16_t s_perm[NPERM * N];
   __shared__ LdsBins<LOGK> s_bins;
+  __shared__ SortStage<N, HM == 5> s_sort;
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palette[i];
   for (uint32_t i = tid; i < (uint32_t)(NPERM * N); i += N) s_perm[i] = g.perms[i];
-  if constexpr (HM == 4) lds_bins_clear(s_bins);
+  if constexpr (HM == 4) {
+    lds_bins_clear(s_bins);
+    lds_bins_palette(s_bins, g.palette);
+  }
   __syncthreads();
 
   unsigned long long drawn = 0;
@@ -102,8 +177,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __
       x = o.x; y = o.y; c = o.c;
       if (g.do_draw) {
         const Located L = locate(blk, o, lrng, g.ss, g.W, g.H);
-        if constexpr (HM == 4) bin_lds(s_bins, g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal, hits);
-        else bin_sample<HM>(g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal);
+        if constexpr (HM == 4) {
+          bin_lds(s_bins, g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal, hits);
+        } else if constexpr (HM == 5) {
+          const uint32_t slot = (it % SORT_STAGE_ITERS) * N + tid;
+          s_sort.bin[slot] = L.valid ? L.bidx : SORT_NONE;
+          s_sort.pid[slot] = (uint8_t)L.pidx;
+        } else {
+          bin_sample<HM>(g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal);
+        }
         drawn += (unsigned long long)__popcll(__ballot(L.valid));
       }
       const uint32_t row = grng.next_a() & (uint32_t)(NPERM - 1);
@@ -112,6 +194,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __
       s_x[b][dst] = x; s_y[b][dst] = y; s_c[b][dst] = c;
       lds_barrier();
       x = s_x[b][tid]; y = s_y[b][tid]; c = s_c[b][tid];
+      if constexpr (HM == 5) {
+        // the exchange barrier above also orders the staging writes before the flush
+        if (g.do_draw && ((it % SORT_STAGE_ITERS) == SORT_STAGE_ITERS - 1 || it + 1 == g.n_iter))
+          sort_flush(s_sort, ((it % SORT_STAGE_ITERS) + 1) * (uint32_t)N, g.sort, g.bins, s_pal);
+      }
     }
     g.pos[pid] = make_float4(x, y, c, 0.0f);
     // The next batch's first exchange writes buffer 0, which slower waves may
@@ -160,7 +247,8 @@ static hipError_t launch_fast_w(const FastArgs& a, uint64_t n, int hm, hipStream
   switch (hm) {
     case 2: return launch_fast_k<W, 2, 0>(a, n, s);
     case 3: return launch_fast_k<W, 3, 0>(a, n, s);
-    case 4: return launch_fast_k<W, 4, (W == 8 ? 11 : 10)>(a, n, s);
+    case 4: return launch_fast_k<W, 4, 10>(a, n, s);
+    case 5: return launch_fast_k<W, 5, 0>(a, n, s);
     default: return launch_fast_k<W, 1, 0>(a, n, s);
   }
 }

@@ -1,0 +1,44 @@
+// Sort-based histogram accumulation (RF_HIST_SORT): shared definitions.
+//
+// Instead of one memory-side atomic per sample, the chaos-game kernel writes
+// 4-byte records into coarse buckets of 2^19 bins, a second kernel splits
+// each bucket into 256 tiles of 2^11 bins, and a third accumulates every tile
+// in LDS and adds it to the histogram with one plain read-modify-write per
+// touched bin.  Records carry no weight, so the mode needs unit opacities;
+// the LDS sums are fixed point, so palette values must lie in [0, 1].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rfk {
+
+constexpr int SORT_COARSE_SHIFT = 19;                                        // bins per bucket: 2^19
+constexpr int SORT_TILE_SHIFT = 11;                                          // bins per tile: 2^11
+constexpr int SORT_TILES_PER_BUCKET = 1 << (SORT_COARSE_SHIFT - SORT_TILE_SHIFT);  // 256
+constexpr int SORT_MAX_BUCKETS = 256;                                        // 2^27 bins
+constexpr int SORT_STAGE_ITERS = 4;                                          // iterations staged in LDS per flush
+constexpr int SORT_CHUNK = 8192;                                             // records per rebucketing workgroup
+constexpr int SORT_SEG = 131072;                                             // records per tile-summing workgroup
+constexpr uint32_t SORT_NONE = 0xffffffffu;
+
+struct SortArgs {
+  uint32_t* cbuf;          // bucket records: ((bin mod 2^19) << 8) | palette index
+  uint32_t* tbuf;          // tile records:   ((bin mod 2^11) << 8) | palette index
+  uint32_t* bcur;          // per bucket: records offered (may exceed the capacity)
+  uint32_t* tcur;          // per tile: records offered
+  const uint32_t* boff;    // per bucket: first record in cbuf
+  const uint32_t* bcap;    // per bucket: capacity
+  const uint32_t* toff;    // per tile: first record in tbuf
+  const uint32_t* tcap;    // per tile: capacity
+  const uint32_t* chunks;  // rebucketing work: (bucket << 16) | chunk index, one per workgroup
+  const uint32_t* segs;    // tile work: (tile << 16) | segment index, one per workgroup
+  uint32_t n_buckets;
+  uint32_t n_tiles;        // n_buckets * SORT_TILES_PER_BUCKET
+  uint32_t n_chunks, n_segs;
+};
+
+hipError_t launch_sort_reset(const SortArgs& a, hipStream_t s);
+hipError_t launch_sort_rebucket(const SortArgs& a, float* bins, const float4* palette, hipStream_t s);
+hipError_t launch_sort_tiles(const SortArgs& a, float* bins, uint32_t n_bins, const float4* palette, hipStream_t s);
+
+}  // namespace rfk
