@@ -471,9 +471,10 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
   uint64_t cb = 0, ct = 0;
   max_bcap = 0;
   for (uint32_t b = 0; b < nb; ++b) {
-    const uint64_t c = (uint64_t)std::ceil((double)S * f->sort_bfrac[b] * 1.2) + 2048;
+    uint64_t c = (uint64_t)std::ceil((double)S * f->sort_bfrac[b] * 1.2) + 2048;
+    c = (c + 3) & ~3ull;  // every bucket's records start 16-byte aligned
     boff[b] = (uint32_t)cb;
-    bcap[b] = (uint32_t)std::min<uint64_t>(c, 0x7fffffffull);
+    bcap[b] = (uint32_t)std::min<uint64_t>(c, 0x7ffffffcull);
     cb += bcap[b];
     max_bcap = std::max(max_bcap, bcap[b]);
   }

@@ -181,13 +181,13 @@ static hipError_t launch_batch_k(const BatchArgs& a, uint32_t max_grid, hipStrea
 hipError_t batch_prepare(int waves, int hist_mode) {
   int r = 0;
   const bool lds = hist_mode == 4;
-  if (waves == 8) return lds ? resident_k<8, 4, 10>(&r) : resident_k<8, 1, 0>(&r);
+  if (waves == 8) return lds ? resident_k<8, 4, 11>(&r) : resident_k<8, 1, 0>(&r);
   return lds ? resident_k<4, 4, 10>(&r) : resident_k<4, 1, 0>(&r);
 }
 
 hipError_t launch_batch(const BatchArgs& a, int waves, int hist_mode, uint32_t max_grid, hipStream_t s) {
   const bool lds = hist_mode == 4;
-  if (waves == 8) return lds ? launch_batch_k<8, 4, 10>(a, max_grid, s) : launch_batch_k<8, 1, 0>(a, max_grid, s);
+  if (waves == 8) return lds ? launch_batch_k<8, 4, 11>(a, max_grid, s) : launch_batch_k<8, 1, 0>(a, max_grid, s);
   return lds ? launch_batch_k<4, 4, 10>(a, max_grid, s) : launch_batch_k<4, 1, 0>(a, max_grid, s);
 }
 

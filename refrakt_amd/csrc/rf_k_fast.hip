@@ -24,24 +24,30 @@ using namespace detail;
 // streams are keyed by particle, wave-of-batch and batch index, never by the
 // workgroup, so the results do not depend on the grid size.
 //
-// HM 1/2/3 bin straight to the histogram and launch one workgroup per batch;
-// HM 4 (LDS bin cache) launches one workgroup per resident slot so that each
-// workgroup's LDS table spans many batches before it is flushed.
-// HIST_SORT staging: the samples of up to SORT_STAGE_ITERS iterations wait in
+// HM 1/2/3/5 launch one workgroup per batch; HM 4 (LDS bin cache) launches
+// one workgroup per resident slot so that each workgroup's LDS table spans
+// many batches before it is flushed.
+// HIST_SORT staging: the samples of up to SORT_STAGE_SLOTS / N iterations wait in
 // LDS (one fixed slot per lane and iteration, no append atomics).  A flush
 // ranks them by bucket (counting sort), reserves one run per non-empty bucket
 // with one global atomic, places the records in bucket order in LDS and
 // writes them out so that neighbouring lanes store neighbouring addresses.
+template <int N>
+constexpr int sort_stage_iters() { return SORT_STAGE_SLOTS / N; }
+
 template <int N, bool ON>
 struct SortStage {
-  uint32_t bin[SORT_STAGE_ITERS * N];
-  uint32_t rec[SORT_STAGE_ITERS * N];   // bucket-ordered records
-  uint16_t rank[SORT_STAGE_ITERS * N];
-  uint8_t pid[SORT_STAGE_ITERS * N];
-  uint8_t bkt[SORT_STAGE_ITERS * N];    // bucket of rec[j]
+  static constexpr int S = SORT_STAGE_SLOTS;
+  uint32_t bin[S];
+  uint32_t rec[S];       // bucket-ordered records
+  uint16_t rank[S];
+  uint8_t pid[S];
+  uint8_t bkt[S];        // bucket of rec[j]
   uint32_t cnt[SORT_MAX_BUCKETS];
   uint32_t off[SORT_MAX_BUCKETS];
   uint32_t gb[SORT_MAX_BUCKETS];
+  uint32_t boff[SORT_MAX_BUCKETS];   // bucket offsets and capacities, staged once per workgroup
+  uint32_t bcap[SORT_MAX_BUCKETS];
   uint32_t total;
 };
 template <int N>
@@ -85,8 +91,8 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_sl
       const uint32_t b = st.bkt[j];
       const uint32_t rec = st.rec[j];
       const uint32_t dst = st.gb[b] + (j - st.off[b]);
-      if (dst < a.bcap[b]) {
-        a.cbuf[a.boff[b] + dst] = rec;
+      if (dst < st.bcap[b]) {
+        a.cbuf[st.boff[b] + dst] = rec;
       } else {  // over the bucket's capacity: straight to the histogram
         ovf = true;
         bin = (b << SORT_COARSE_SHIFT) | (rec >> 8);
@@ -117,6 +123,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __
   if constexpr (HM == 4) {
     lds_bins_clear(s_bins);
     lds_bins_palette(s_bins, g.palette);
+  }
+  if constexpr (HM == 5) {
+    if (g.do_draw) {
+      for (uint32_t b = tid; b < g.sort.n_buckets; b += N) {
+        s_sort.boff[b] = g.sort.boff[b];
+        s_sort.bcap[b] = g.sort.bcap[b];
+      }
+    }
   }
   __syncthreads();
 
@@ -180,7 +194,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __
         if constexpr (HM == 4) {
           bin_lds(s_bins, g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal, hits);
         } else if constexpr (HM == 5) {
-          const uint32_t slot = (it % SORT_STAGE_ITERS) * N + tid;
+          const uint32_t slot = (it % sort_stage_iters<N>()) * N + tid;
           s_sort.bin[slot] = L.valid ? L.bidx : SORT_NONE;
           s_sort.pid[slot] = (uint8_t)L.pidx;
         } else {
@@ -196,8 +210,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __
       x = s_x[b][tid]; y = s_y[b][tid]; c = s_c[b][tid];
       if constexpr (HM == 5) {
         // the exchange barrier above also orders the staging writes before the flush
-        if (g.do_draw && ((it % SORT_STAGE_ITERS) == SORT_STAGE_ITERS - 1 || it + 1 == g.n_iter))
-          sort_flush(s_sort, ((it % SORT_STAGE_ITERS) + 1) * (uint32_t)N, g.sort, g.bins, s_pal);
+        constexpr uint32_t SI = sort_stage_iters<N>();
+        if (g.do_draw && ((it % SI) == SI - 1 || it + 1 == g.n_iter))
+          sort_flush(s_sort, ((it % SI) + 1) * (uint32_t)N, g.sort, g.bins, s_pal);
       }
     }
     g.pos[pid] = make_float4(x, y, c, 0.0f);
@@ -224,7 +239,8 @@ static hipError_t launch_fast_k(const FastArgs& a0, uint64_t n, hipStream_t s) {
   a.n_batches = (uint32_t)(n / (64u * W));
   uint32_t grid = a.n_batches;
   if (HM == 4) {
-    // one workgroup per resident slot (cached per instantiation)
+    // one workgroup per resident slot (cached per instantiation), so that each
+    // LDS table spans many batches before it is flushed
     static int resident = 0;
     if (!resident) {
       int dev = 0, cus = 0, per_cu = 0;

@@ -20,17 +20,18 @@ __global__ void k_sort_reset(SortArgs a) {
 // tile in LDS, one reservation atomic per non-empty tile, then every record
 // is written to its tile's run.  Records that do not fit in their tile's
 // capacity are added to the histogram directly.
+constexpr int RB_PER = SORT_CHUNK / 256;  // records per thread
+
 __global__ __launch_bounds__(256) void k_sort_rebucket(SortArgs a, float* __restrict__ bins,
                                                        const float4* __restrict__ palette) {
   static_assert(SORT_TILES_PER_BUCKET <= 4 * WAVE, "scan width");
   static_assert(SORT_TILES_PER_BUCKET <= 256, "tile index in a byte");
-  // The chunk is read twice from memory (ranking, then placing) instead of
-  // being kept in LDS, which leaves room for two workgroups per CU.
+  static_assert(RB_PER % 4 == 0, "16-byte loads");
   __shared__ uint32_t s_out[SORT_CHUNK];   // tile-ordered records
-  __shared__ uint16_t s_rank[SORT_CHUNK];
   __shared__ uint8_t s_tile[SORT_CHUNK];   // tile (within the bucket) of s_out[j]
   __shared__ uint32_t s_cnt[SORT_TILES_PER_BUCKET], s_off[SORT_TILES_PER_BUCKET], s_gb[SORT_TILES_PER_BUCKET];
   __shared__ uint32_t s_total;
+  __shared__ uint32_t s_toff[SORT_TILES_PER_BUCKET], s_tcap[SORT_TILES_PER_BUCKET];
   __shared__ float4 s_pal[256];
   const uint32_t tid = threadIdx.x;
   const uint32_t job = a.chunks[blockIdx.x];
@@ -41,14 +42,33 @@ __global__ __launch_bounds__(256) void k_sort_rebucket(SortArgs a, float* __rest
   const uint32_t start = (job & 0xffffu) * (uint32_t)SORT_CHUNK;
   if (start >= count) return;  // workgroup-uniform
   const uint32_t n = (count - start) < (uint32_t)SORT_CHUNK ? (count - start) : (uint32_t)SORT_CHUNK;
+  // bucket offsets and SORT_CHUNK are multiples of 4 records: 16-byte aligned
   const uint32_t* src = a.cbuf + a.boff[b] + start;
-  if (tid < (uint32_t)SORT_TILES_PER_BUCKET) s_cnt[tid] = 0u;
-  s_pal[tid] = palette[tid];
-  __syncthreads();
-  for (uint32_t i = tid; i < n; i += 256) {
-    const uint32_t r = src[i];
-    s_rank[i] = (uint16_t)atomicAdd(&s_cnt[r >> (8 + SORT_TILE_SHIFT)], 1u);
+  const uint4* src4 = reinterpret_cast<const uint4*>(src);
+  if (tid < (uint32_t)SORT_TILES_PER_BUCKET) {
+    s_cnt[tid] = 0u;
+    s_toff[tid] = a.toff[b * SORT_TILES_PER_BUCKET + tid];
+    s_tcap[tid] = a.tcap[b * SORT_TILES_PER_BUCKET + tid];
   }
+  s_pal[tid] = palette[tid];
+  // every thread keeps its RB_PER records in registers between the passes
+  uint32_t rec[RB_PER];
+#pragma unroll
+  for (int k = 0; k < RB_PER / 4; ++k) {
+    const uint32_t i4 = (uint32_t)k * 256u + tid, i = 4u * i4;
+    if (i + 3u < n) {
+      const uint4 q = src4[i4];
+      rec[4 * k] = q.x; rec[4 * k + 1] = q.y; rec[4 * k + 2] = q.z; rec[4 * k + 3] = q.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rec[4 * k + e] = (i + (uint32_t)e) < n ? src[i + e] : SORT_NONE;
+    }
+  }
+  __syncthreads();
+  uint32_t rank[RB_PER];
+#pragma unroll
+  for (int m = 0; m < RB_PER; ++m)
+    rank[m] = rec[m] != SORT_NONE ? atomicAdd(&s_cnt[rec[m] >> (8 + SORT_TILE_SHIFT)], 1u) : 0u;
   __syncthreads();
   wave0_exclusive_scan<4>(s_cnt, s_off, SORT_TILES_PER_BUCKET, &s_total);
   if (tid < (uint32_t)SORT_TILES_PER_BUCKET) {
@@ -56,12 +76,14 @@ __global__ __launch_bounds__(256) void k_sort_rebucket(SortArgs a, float* __rest
     s_gb[tid] = c ? atomicAdd(a.tcur + b * SORT_TILES_PER_BUCKET + tid, c) : 0u;
   }
   __syncthreads();
-  for (uint32_t i = tid; i < n; i += 256) {
-    const uint32_t r = src[i];
-    const uint32_t f = r >> (8 + SORT_TILE_SHIFT);
-    const uint32_t j = s_off[f] + s_rank[i];
-    s_out[j] = r;
-    s_tile[j] = (uint8_t)f;
+#pragma unroll
+  for (int m = 0; m < RB_PER; ++m) {
+    if (rec[m] != SORT_NONE) {
+      const uint32_t f = rec[m] >> (8 + SORT_TILE_SHIFT);
+      const uint32_t j = s_off[f] + rank[m];
+      s_out[j] = rec[m];
+      s_tile[j] = (uint8_t)f;
+    }
   }
   __syncthreads();
   const uint32_t total = s_total;
@@ -73,11 +95,10 @@ __global__ __launch_bounds__(256) void k_sort_rebucket(SortArgs a, float* __rest
     if (j < total) {
       const uint32_t f = s_tile[j];
       const uint32_t r = s_out[j];
-      const uint32_t t = b * SORT_TILES_PER_BUCKET + f;
       const uint32_t local = r >> 8;
       const uint32_t dst = s_gb[f] + (j - s_off[f]);
-      if (dst < a.tcap[t]) {
-        a.tbuf[a.toff[t] + dst] = ((local & ((1u << SORT_TILE_SHIFT) - 1u)) << 8) | (r & 255u);
+      if (dst < s_tcap[f]) {
+        a.tbuf[s_toff[f] + dst] = ((local & ((1u << SORT_TILE_SHIFT) - 1u)) << 8) | (r & 255u);
       } else {
         ovf = true;
         bin = (b << SORT_COARSE_SHIFT) | local;
