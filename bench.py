@@ -21,10 +21,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+KERNELS = {1: "k_fast<quad>", 2: "k_fast<wave-aggregated>", 3: "k_fast<naive>", 4: "k_fast<LDS bin cache>",
+           5: "k_fast<sort staging> + k_sort_rebucket + k_sort_tiles"}
 # HBM bytes per draw launch from rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (scale factors from a 512 MiB copy), profiles/k_fast_cfg2_hbm_counters.md.
 # Keyed by (config, hist mode, waves per block); other keys report null.
-MEASURED_TRAFFIC = {("cfg2", 0, 8): 8.87e9, ("cfg2", 4, 8): 8.87e9, ("cfg2", 1, 4): 9.06e9}
+MEASURED_TRAFFIC = {("cfg2", 0, 8): 5.52e9, ("cfg2", 5, 8): 5.52e9, ("cfg2", 4, 8): 8.76e9, ("cfg2", 1, 4): 9.06e9}
 
 
 def parse():
@@ -262,14 +264,15 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
                 "traffic": MEASURED_TRAFFIC.get((args.config, args.hist_mode, args.waves or 8)),
-                "kernel": "k_fast", "kernel_avg_ms": round(k_avg, 3),
+                "kernel": KERNELS.get(mode_used, "k_fast"), "kernel_avg_ms": round(k_avg, 3),
                 "atomic": {"achieved": round(atomic_g, 2), "peak": round(peak_atomic, 2) if peak_atomic else None,
                            "unit": "G f32 atomic adds/s",
                            "frac": round(atomic_g / peak_atomic, 4) if peak_atomic else None},
                 "note": "algorithmic bytes = 32 B per binned sample (16 B RGBA read + write) + 24 B per particle; "
-                        "the binding limit is the memory-side atomic request rate: one 16-byte request per sample "
-                        "that misses the LDS bin cache; atomic.achieved counts every binned sample's 4 adds, "
-                        "atomic.peak = live probe of the 16-byte request shape"}
+                        "kernel_avg_ms = HIP-event time of one draw_to_bins call (all its launches); traffic from "
+                        "rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE over the draw's kernels; atomic.achieved counts "
+                        "every binned sample's 4 adds against the live probe of 16-byte memory-side atomics "
+                        "(above 1: most samples never reach a memory-side atomic)"}
     # DE + tonemap of the finished histogram (HBM-bound pass), timed separately
     out = torch.empty((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
     de = rf.DEParams(wl.flame.estimator_radius, wl.flame.estimator_min, wl.flame.estimator_curve)
