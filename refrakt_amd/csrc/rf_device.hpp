@@ -130,7 +130,7 @@ __device__ __forceinline__ void rf_sincos(float x, float* s, float* c) {
 }
 __device__ __forceinline__ float rf_sin(float x) { float s, c; rf_sincos(x, &s, &c); return s; }
 __device__ __forceinline__ float rf_cos(float x) { float s, c; rf_sincos(x, &s, &c); return c; }
-__device__ __forceinline__ float rf_tan(float x) { float s, c; rf_sincos(x, &s, &c); return s / c; }
+__device__ __forceinline__ float rf_tan(float x) { float s, c; rf_sincos(x, &s, &c); return s * __builtin_amdgcn_rcpf(c); }
 // exp, log and pow through the hardware base-2 instructions; GLSL defines
 // pow(x, y) as exp2(y * log2(x)) for x > 0.
 __device__ __forceinline__ float rf_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
@@ -139,8 +139,50 @@ __device__ __forceinline__ float rf_exp(float x) { return rf_exp2(x * 1.44269504
 __device__ __forceinline__ float rf_log(float x) { return rf_log2(x) * 0.693147180559945309f; }
 __device__ __forceinline__ float rf_log10(float x) { return rf_log2(x) * 0.301029995663981195f; }
 __device__ __forceinline__ float rf_pow(float x, float y) { return rf_exp2(y * rf_log2(x)); }
-__device__ __forceinline__ float rf_sinh(float x) { const float e = rf_exp(x); return 0.5f * (e - 1.0f / e); }
-__device__ __forceinline__ float rf_cosh(float x) { const float e = rf_exp(x); return 0.5f * (e + 1.0f / e); }
+__device__ __forceinline__ float rf_sinh(float x) {
+  const float e = rf_exp(x);
+  return 0.5f * (e - __builtin_amdgcn_rcpf(e));
+}
+__device__ __forceinline__ float rf_cosh(float x) {
+  const float e = rf_exp(x);
+  return 0.5f * (e + __builtin_amdgcn_rcpf(e));
+}
+// Division and square root of the variation interpreter: the hardware
+// reciprocal (1 ulp) times the dividend, and the hardware square root, both
+// within the 2.5 ulp GLSL allows for division and sqrt.  The compiler's own
+// 2.5 ulp division rescales both operands with frexp/ldexp to cover denormal
+// and huge divisors (8 instructions per division instead of 2); chaos-game
+// coordinates stay within +-1e10 (larger ones are reset), so the rescaling
+// buys nothing here.
+__device__ __forceinline__ float rf_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rf_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// atan2(y, x): the quotient t = min(|x|,|y|) / max(|x|,|y|) in [0, 1], an odd
+// polynomial for atan(t) (relative-error minimax fit of atan(t)/t in t^2 on
+// [0, 1], 8 terms, max error 2.9 ulp in float32 evaluation), then the octant
+// reflections pi/2 - r and pi - r and the sign of y.  The sign bit of x picks
+// the pi - r branch, so atan2(+-0, -0) = +-pi as in C.  About 20 instructions
+// against about 40 for the device library's atan2f, whose infinity handling
+// the interpreter does not need (coordinates are finite).
+__device__ __forceinline__ float rf_atan2(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  const float t = mx > 0.0f ? mn * rf_rcp(mx) : 0.0f;
+  const float z = t * t;
+  float p = -4.693436035e-03f;
+  p = fmaf(p, z, 2.425300952e-02f);
+  p = fmaf(p, z, -5.948731153e-02f);
+  p = fmaf(p, z, 9.914363580e-02f);
+  p = fmaf(p, z, -1.401951006e-01f);
+  p = fmaf(p, z, 1.996973005e-01f);
+  p = fmaf(p, z, -3.333199132e-01f);
+  p = fmaf(p, z, 9.999999011e-01f);
+  float r = t * p;
+  if (ay > ax) r = PI_2 - r;
+  if (__float_as_uint(x) >> 31) r = PI - r;
+  return __uint_as_float((__float_as_uint(r) & 0x7fffffffu) | (__float_as_uint(y) & 0x80000000u));
+}
+
 // Correctly rounded sqrt and division for the few places whose result must
 // match the CPU bit for bit (the library is built with fast fp32 div/sqrt).
 // They go through binary64: rounding the correctly rounded double result to
@@ -293,11 +335,14 @@ __device__ __forceinline__ XOut apply_xform(BP blk, int xi, float x0, float y0,
   }
   const uint32_t m = xf->precalc;
   if (m & (PC_RSQ | PC_R)) pc.rsq = x * x + y * y;
-  if (m & PC_R) pc.r = sqrtf(pc.rsq);
-  if (m & PC_A) pc.a = atan2f(x, y);
-  if (m & PC_PHI) pc.phi = atan2f(y, x);
-  if (m & PC_SINA) pc.sina = x / pc.r;
-  if (m & PC_COSA) pc.cosa = y / pc.r;
+  if (m & PC_R) pc.r = rf_sqrt(pc.rsq);
+  if (m & PC_A) pc.a = rf_atan2(x, y);
+  if (m & PC_PHI) pc.phi = rf_atan2(y, x);
+  if (m & (PC_SINA | PC_COSA)) {
+    const float ir = rf_rcp(pc.r);
+    pc.sina = x * ir;
+    pc.cosa = y * ir;
+  }
   if (m & (PC_SINR | PC_COSR)) rf_sincos(pc.r, &pc.sinr, &pc.cosr);
   float rx = 0.0f, ry = 0.0f;
   for (uint32_t k = npre; k < nt; ++k) {
@@ -305,23 +350,21 @@ __device__ __forceinline__ XOut apply_xform(BP blk, int xi, float x0, float y0,
 #pragma unroll
     for (int j = 0; j < MAX_PRM; ++j) P[j] = T[k].prm[j];
     const float w = T[k].weight;
+    float ox, oy;
+    eval_var(T[k].var, x, y, pc, w, A, P, rng, ox, oy);
+    if (!(T[k].flags & TF_NO_WEIGHT_MUL)) { ox = w * ox; oy = w * oy; }
+    if (k == npre) { rx = ox; ry = oy; }
+    else { rx = rx + ox; ry = ry + oy; }
     // Everything a variation computes from x, y, the precalcs or the affine is
     // invariant in this loop, and loop-invariant code motion would hoist it out
     // of the switch -- i.e. evaluate sin(x), sin(rsq), exp(x - 1), ... of all 76
     // variations on every xform application.  An empty asm that claims to
-    // rewrite these registers keeps each case's work inside its case.
-    float xk = x, yk = y, Ak[6];
-    Pre pk = pc;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) Ak[j] = A[j];
-    asm volatile("" : "+v"(xk), "+v"(yk), "+v"(pk.r), "+v"(pk.rsq), "+v"(pk.a), "+v"(pk.phi), "+v"(pk.sina),
-                 "+v"(pk.cosa), "+v"(pk.sinr), "+v"(pk.cosr));
-    asm volatile("" : "+s"(Ak[0]), "+s"(Ak[1]), "+s"(Ak[2]), "+s"(Ak[3]), "+s"(Ak[4]), "+s"(Ak[5]));
-    float ox, oy;
-    eval_var(T[k].var, xk, yk, pk, w, Ak, P, rng, ox, oy);
-    if (!(T[k].flags & TF_NO_WEIGHT_MUL)) { ox = w * ox; oy = w * oy; }
-    if (k == npre) { rx = ox; ry = oy; }
-    else { rx = rx + ox; ry = ry + oy; }
+    // rewrite these values at the end of every term makes them loop-carried,
+    // which keeps each case's work inside its case; it rewrites the values in
+    // place, so it costs no register copies.
+    asm volatile("" : "+v"(x), "+v"(y), "+v"(pc.r), "+v"(pc.rsq), "+v"(pc.a), "+v"(pc.phi), "+v"(pc.sina),
+                 "+v"(pc.cosa), "+v"(pc.sinr), "+v"(pc.cosr));
+    asm volatile("" : "+s"(A[0]), "+s"(A[1]), "+s"(A[2]), "+s"(A[3]), "+s"(A[4]), "+s"(A[5]));
   }
   if (xf->flags & XF_HAS_POST) {
     const float q0 = xf->post[0], q1 = xf->post[1], q2 = xf->post[2];

@@ -3,6 +3,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -82,3 +83,31 @@ def test_create_rejects_bad_sim_parameters(rf):
         assert L.rf_create(api.C.byref(sp), api.C.byref(out)) == -1  # RF_ERR_ARG
         assert word in L.rf_last_error(None).decode()
         assert not out.value
+
+
+def test_division_rewrite_keeps_c_grouping():
+    """Variation sources divide with `/`; the generator turns every a / b into
+    a * rf_rcp(b).  `/` and `*` share precedence and associativity, so only
+    the divisor's extent matters: a unary expression (casts, prefix signs,
+    calls, subscripts, parenthesised groups), nested divisions included."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen_variations", os.path.join(ROOT, "refrakt_amd",
+                                                                                 "gen_variations.py"))
+    g = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(g)
+    cases = {
+        "ox = x / d; oy = y / d;": "ox = x * rf_rcp(d); oy = y * rf_rcp(d);",
+        "a * sinf(t) / PI": "a * sinf(t) * rf_rcp(PI)",
+        "k = 2.0f / (r + 1.0f);": "k = 2.0f * rf_rcp((r + 1.0f));",
+        "m / -b[2] + c": "m * rf_rcp(-b[2]) + c",
+        "p / (float)k / q": "p * rf_rcp((float)k) * rf_rcp(q)",
+        "e / fmaxf(u * u, 1e-20f)": "e * rf_rcp(fmaxf(u * u, 1e-20f))",
+        "c / (1.0f + c * x / r) / r": "c * rf_rcp((1.0f + c * x * rf_rcp(r))) * rf_rcp(r)",
+        "(int)((rr + dx) / (2.0f * dx))": "(int)((rr + dx) * rf_rcp((2.0f * dx)))",
+        "y / 2.0f": "y * rf_rcp(2.0f)",
+        "v / 1.5e-3f;": "v * rf_rcp(1.5e-3f);",
+    }
+    for src, want in cases.items():
+        assert g.rcp_divisions(src) == want, (src, g.rcp_divisions(src))
+    with pytest.raises(ValueError):
+        g.rcp_divisions("x /= d;")
