@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-probe", action="store_true")
     ap.add_argument("--no-psnr", action="store_true", help="skip the time-to-40-dB measurement")
+    ap.add_argument("--no-programs", action="store_true",
+                    help="run the precompiled interpreter instead of the flame's compiled program")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -67,7 +69,7 @@ def cpu_baseline(wl, budget_s):
                       f"{W // 4}x{H // 4} histogram; {n} binned in {dt:.1f} s"}
 
 
-def time_to_psnr(rf, wl, target_db, hist_mode, waves):
+def time_to_psnr(rf, wl, target_db, hist_mode, waves, programs=True):
     """Plain method (more samples + DE): wall time from a fresh flame until the
     normalised log-density image of the density-estimated histogram reaches
     target_db against the FP64 oracle's histogram image at 2^32 samples
@@ -84,7 +86,7 @@ def time_to_psnr(rf, wl, target_db, hist_mode, waves):
         return None
     ref_t = torch.from_numpy(ref).cuda()
     ctx = rf.Context(wl.particles, wl.temporal_samples, 1024, seed=777, engine=rf.RF_ENGINE_FAST,
-                     hist_mode=hist_mode, block_waves=waves)
+                     hist_mode=hist_mode, block_waves=waves, flame_programs=programs)
     fh = ctx.flame(wl.flame)
     hist_t = torch.zeros((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
     hist = ctx.histogram(wl.width, wl.height, device_ptr=hist_t.data_ptr())
@@ -224,7 +226,7 @@ def main():
     wl = configs.WORKLOADS[args.config]()
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     ctx = rf.Context(wl.particles, wl.temporal_samples, 1024, seed=12345, engine=rf.RF_ENGINE_FAST,
-                     hist_mode=args.hist_mode, block_waves=args.waves)
+                     hist_mode=args.hist_mode, block_waves=args.waves, flame_programs=not args.no_programs)
     fh = ctx.flame(wl.flame)
     hist = ctx.histogram(wl.width, wl.height)
     fh.warmup(wl.warmup_passes)
@@ -242,6 +244,7 @@ def main():
     t1 = time.perf_counter()
     wall = t1 - t0
     mode_used, lds_hits = fh.hist_mode()
+    prog_used, prog_compile_s = fh.program_info()
     value = drawn / wall
     attempted = wl.particles * wl.draw_passes * args.steps
 
@@ -302,12 +305,13 @@ def main():
                                f"{wl.draw_passes} passes/step",
                    "binned_samples": drawn, "attempted_samples": attempted, "hist_mode": args.hist_mode,
                    "hist_mode_used": mode_used, "lds_hit_fraction": round(lds_hits, 4) if lds_hits >= 0 else None,
-                   "block_waves": args.waves or 8, "parallelism": "single GPU"},
+                   "block_waves": args.waves or 8, "parallelism": "single GPU",
+                   "flame_program": {"used": bool(prog_used == 1), "compile_s": round(prog_compile_s, 3)}},
         "roofline": roofline,
         "de_tonemap_ms": round(de_ms, 3),
     }
     if not args.no_psnr:
-        line["time_to_40db"] = time_to_psnr(rf, wl, 40.0, args.hist_mode, args.waves)
+        line["time_to_40db"] = time_to_psnr(rf, wl, 40.0, args.hist_mode, args.waves, not args.no_programs)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(line))

@@ -364,6 +364,31 @@ int rf_flame_hist_mode(const rf_flame* flame, double* lds_hit_fraction);
 /* Fraction of the binned samples of the flame's last RF_HIST_SORT draw that
  * overflowed the bucket/tile capacities (and were added with atomics), or -1. */
 double rf_flame_sort_overflow(const rf_flame* flame);
+/* --------------------------------------------------- flame programs */
+/* The reference generates and compiles a GLSL program per flame
+ * (src/variation_table.cpp:217-265, compiled in src/flame.cpp:285-288).  The
+ * fast engine's precompiled kernels interpret any flame; with flame programs
+ * on (the default; environment RF_FLAME_PROGRAMS=0 turns them off for new
+ * contexts) the first warmup or draw of a flame compiles the same kernel with
+ * the flame's structure fixed (hiprtc, cached per structure and device for the
+ * life of the process) and later launches run it.  Results are the same bits
+ * as the interpreter's.  enable: 1 on, 0 off. */
+int rf_set_flame_programs(rf_ctx* ctx, int32_t enable);
+/* 1 if the flame's last fast-engine launch ran a flame program, 0 if it was
+ * interpreted, -1 if none ran yet; *compile_seconds (may be NULL) = seconds
+ * spent compiling its program in this process (0 when it came from the cache). */
+int rf_flame_program_info(const rf_flame* flame, double* compile_seconds);
+/* HIP source of the flame program of desc with one entry point for
+ * (block_waves, hist_mode) (hist_mode 1..5).  Writes at most cap bytes
+ * (NUL-terminated) and returns the full length + 1, or < 0 on error. */
+int rf_flame_program_source(const rf_flame_desc* desc, int32_t block_waves, int32_t hist_mode, char* buf,
+                            size_t cap);
+/* Compiles that source for arch (e.g. "gfx950") without a device: the code
+ * object goes to code (cap bytes; *size = its size), the compiler log to log.
+ * *seconds (may be NULL) = compile time.  RF_OK, or RF_ERR_ARG / RF_ERR_HIP. */
+int rf_flame_program_build(const rf_flame_desc* desc, int32_t block_waves, int32_t hist_mode, const char* arch,
+                           void* code, size_t cap, size_t* size, char* log, size_t log_len, double* seconds);
+
 /* Screen-space affine used by draw_to_bins (src/flame.cpp:289-296). */
 int rf_screen_affine(const rf_flame_desc* desc, uint32_t width, uint32_t height, float out[6]);
 

@@ -164,6 +164,12 @@ _SIGS = {
     "rf_rgba8_device": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "rf_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),
     "rf_screen_affine": (C.c_int, [C.POINTER(FlameDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
+    "rf_set_flame_programs": (C.c_int, [_P, C.c_int32]),
+    "rf_flame_program_info": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "rf_flame_program_source": (C.c_int, [C.POINTER(FlameDesc), C.c_int32, C.c_int32, C.c_char_p, C.c_size_t]),
+    "rf_flame_program_build": (C.c_int, [C.POINTER(FlameDesc), C.c_int32, C.c_int32, C.c_char_p, C.c_void_p,
+                                         C.c_size_t, C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t,
+                                         C.POINTER(C.c_double)]),
     "rf_variation_count": (C.c_int, []),
     "rf_variation_name": (C.c_char_p, [C.c_int]),
     "rf_variation_id": (C.c_int, [C.c_char_p]),
@@ -373,6 +379,35 @@ def screen_affine(flame: Flame, width: int, height: int) -> np.ndarray:
     return out
 
 
+def program_source(flame: "Flame", block_waves: int = 8, hist_mode: int = 5) -> str:
+    """HIP source of the flame program of `flame` (one entry point)."""
+    L = lib()
+    d = flame.to_c()
+    n = L.rf_flame_program_source(C.byref(d), block_waves, hist_mode, None, 0)
+    if n < 0:
+        raise RefraktError(f"rf_flame_program_source failed ({n})")
+    buf = C.create_string_buffer(n)
+    L.rf_flame_program_source(C.byref(d), block_waves, hist_mode, buf, n)
+    return buf.value.decode()
+
+
+def program_build(flame: "Flame", block_waves: int = 8, hist_mode: int = 5, arch: str = "gfx950"):
+    """Compiles the flame program for `arch` without a device; returns (code
+    object bytes, compiler log, seconds)."""
+    L = lib()
+    d = flame.to_c()
+    size = C.c_size_t(0)
+    log = C.create_string_buffer(1 << 16)
+    sec = C.c_double(0.0)
+    cap = 16 << 20
+    code = C.create_string_buffer(cap)
+    rc = L.rf_flame_program_build(C.byref(d), block_waves, hist_mode, arch.encode(), code, cap, C.byref(size), log,
+                                  len(log), C.byref(sec))
+    if rc != RF_OK:
+        raise RefraktError(f"rf_flame_program_build failed ({rc}): {log.value.decode(errors='replace')}")
+    return code.raw[:size.value], log.value.decode(errors="replace"), sec.value
+
+
 def validate(flame: Flame) -> str:
     """Empty string if rf_flame_create would accept the flame, else the reason."""
     d = flame.to_c()
@@ -403,7 +438,7 @@ class Context:
 
     def __init__(self, total_particles: int, temporal_samples: int = 512, shuffle_count: int = 1024,
                  seed: int = 1, device: int = 0, engine: int = RF_ENGINE_FAST, hist_mode: int = RF_HIST_AUTO,
-                 block_waves: int = 0):
+                 block_waves: int = 0, flame_programs: Optional[bool] = None):
         self.L = lib()
         sp = SimParams(total_particles, temporal_samples, shuffle_count, seed, device, engine, hist_mode, block_waves)
         h = C.c_void_p()
@@ -411,6 +446,8 @@ class Context:
         if rc != RF_OK:
             raise RefraktError(f"rf_create failed ({rc}): {self.L.rf_last_error(None).decode()}")
         self.h = h
+        if flame_programs is not None:
+            self.set_flame_programs(flame_programs)
         self.total_particles = total_particles
         self.temporal_samples = temporal_samples
         self.shuffle_count = shuffle_count
@@ -441,6 +478,11 @@ class Context:
     @property
     def stream_ptr(self) -> int:
         return self.L.rf_stream(self.h) or 0
+
+    def set_flame_programs(self, enable: bool):
+        """Fast engine: run per-structure compiled flame programs (True, the
+        default) or the precompiled interpreter (False)."""
+        self.check(self.L.rf_set_flame_programs(self.h, 1 if enable else 0))
 
     def set_engine(self, engine, hist_mode=RF_HIST_QUAD):
         self.check(self.L.rf_set_engine(self.h, engine, hist_mode))
@@ -585,6 +627,13 @@ class FlameHandle:
     def sort_overflow(self) -> float:
         """Overflow fraction of the last sort-mode draw (-1 if none)."""
         return float(self.ctx.L.rf_flame_sort_overflow(self.h))
+
+    def program_info(self):
+        """(1 if the last fast-engine launch ran a flame program, 0 if it was
+        interpreted, -1 if none ran; seconds spent compiling its program)."""
+        sec = C.c_double(0.0)
+        used = self.ctx.L.rf_flame_program_info(self.h, C.byref(sec))
+        return used, sec.value
 
     def param_block(self) -> np.ndarray:
         return param_block(self.flame)

@@ -6,8 +6,10 @@
 // device memory, read with wave-uniform addresses (scalar loads), and each
 // wave runs one xform at a time so the variation `switch` is uniform.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace rfd {
 
@@ -303,12 +305,80 @@ __device__ __forceinline__ void eval_var(uint32_t id, float x, float y, const Pr
 // (dispatch() in the generated GLSL returns vec4(result, color, opacity)).
 struct XOut { float x, y, c, w; };
 
-// Applies xform `xi` of `blk` to (x0, y0, c0).  If `first` the incoming
-// colour is replaced by a random draw made after the variations' draws, as
-// the generated code does on the first run (variation_table.cpp:241-243).
+// The pieces of one xform application, shared by the interpreter
+// (apply_xform: structure read from the Block at run time) and by compiled
+// flame programs (apply_fixed: structure fixed at compile time, coefficients
+// still read from the Block), so both evaluate the same operations in the
+// same order and produce the same bits.
 // Evaluation order follows variation_table.cpp:97-165: fma affine, pre_xform
 // terms added into v, precalcs, the regular terms summed in alphabetical order
 // (separate multiply and add), then the fma post-affine.
+
+// One pre_xform term: its (weighted) result is added into the point.
+template <class Rng, class TP>
+__device__ __forceinline__ void pre_term(uint32_t var, uint32_t flags, TP t, float& x, float& y, const Pre& pc,
+                                         const float* A, Rng& rng) {
+  float P[MAX_PRM];
+#pragma unroll
+  for (int j = 0; j < MAX_PRM; ++j) P[j] = t->prm[j];
+  const float w = t->weight;
+  float ox, oy;
+  eval_var(var, x, y, pc, w, A, P, rng, ox, oy);
+  if (!(flags & TF_NO_WEIGHT_MUL)) { ox = w * ox; oy = w * oy; }
+  x = x + ox;
+  y = y + oy;
+}
+
+// The precalcs named by mask m (PC_* bits); the others stay 0.
+__device__ __forceinline__ void precalc(uint32_t m, float x, float y, Pre& pc) {
+  if (m & (PC_RSQ | PC_R)) pc.rsq = x * x + y * y;
+  if (m & PC_R) pc.r = rf_sqrt(pc.rsq);
+  if (m & PC_A) pc.a = rf_atan2(x, y);
+  if (m & PC_PHI) pc.phi = rf_atan2(y, x);
+  if (m & (PC_SINA | PC_COSA)) {
+    const float ir = rf_rcp(pc.r);
+    pc.sina = x * ir;
+    pc.cosa = y * ir;
+  }
+  if (m & (PC_SINR | PC_COSR)) rf_sincos(pc.r, &pc.sinr, &pc.cosr);
+}
+
+// One regular term: the first one sets the sum, the others add to it.
+template <class Rng, class TP>
+__device__ __forceinline__ void reg_term(uint32_t var, uint32_t flags, TP t, float x, float y, const Pre& pc,
+                                         const float* A, Rng& rng, bool first_term, float& rx, float& ry) {
+  float P[MAX_PRM];
+#pragma unroll
+  for (int j = 0; j < MAX_PRM; ++j) P[j] = t->prm[j];
+  const float w = t->weight;
+  float ox, oy;
+  eval_var(var, x, y, pc, w, A, P, rng, ox, oy);
+  if (!(flags & TF_NO_WEIGHT_MUL)) { ox = w * ox; oy = w * oy; }
+  if (first_term) { rx = ox; ry = oy; }
+  else { rx = rx + ox; ry = ry + oy; }
+}
+
+// Post-affine (when `post`) and colour blend.  If `first` the incoming
+// colour is replaced by a random draw made after the variations' draws, as
+// the generated code does on the first run (variation_table.cpp:241-243).
+template <class Rng, class XP>
+__device__ __forceinline__ XOut finish_xform(XP xf, bool post, float rx, float ry, float c0, bool first, Rng& rng) {
+  if (post) {
+    const float q0 = xf->post[0], q1 = xf->post[1], q2 = xf->post[2];
+    const float q3 = xf->post[3], q4 = xf->post[4], q5 = xf->post[5];
+    float px = fmaf(q0, rx, fmaf(q2, ry, q4));
+    float py = fmaf(q1, rx, fmaf(q3, ry, q5));
+    rx = px; ry = py;
+  }
+  float cin = first ? rng.randf() : c0;
+  XOut o;
+  o.x = rx; o.y = ry;
+  o.c = rf_mix(cin, xf->color, xf->color_speed);
+  o.w = xf->opacity;
+  return o;
+}
+
+// Applies xform `xi` of `blk` to (x0, y0, c0), structure read from the Block.
 template <class Rng, class BP>
 __device__ __forceinline__ XOut apply_xform(BP blk, int xi, float x0, float y0,
                                             float c0, bool first, Rng& rng) {
@@ -322,39 +392,11 @@ __device__ __forceinline__ XOut apply_xform(BP blk, int xi, float x0, float y0,
   const auto* __restrict__ T = blk->terms + tb;
   Pre pc;
   pc.r = pc.rsq = pc.a = pc.phi = pc.sina = pc.cosa = pc.sinr = pc.cosr = 0.0f;
-  for (uint32_t k = 0; k < npre; ++k) {
-    float P[MAX_PRM];
-#pragma unroll
-    for (int j = 0; j < MAX_PRM; ++j) P[j] = T[k].prm[j];
-    const float w = T[k].weight;
-    float ox, oy;
-    eval_var(T[k].var, x, y, pc, w, A, P, rng, ox, oy);
-    if (!(T[k].flags & TF_NO_WEIGHT_MUL)) { ox = w * ox; oy = w * oy; }
-    x = x + ox;
-    y = y + oy;
-  }
-  const uint32_t m = xf->precalc;
-  if (m & (PC_RSQ | PC_R)) pc.rsq = x * x + y * y;
-  if (m & PC_R) pc.r = rf_sqrt(pc.rsq);
-  if (m & PC_A) pc.a = rf_atan2(x, y);
-  if (m & PC_PHI) pc.phi = rf_atan2(y, x);
-  if (m & (PC_SINA | PC_COSA)) {
-    const float ir = rf_rcp(pc.r);
-    pc.sina = x * ir;
-    pc.cosa = y * ir;
-  }
-  if (m & (PC_SINR | PC_COSR)) rf_sincos(pc.r, &pc.sinr, &pc.cosr);
+  for (uint32_t k = 0; k < npre; ++k) pre_term(T[k].var, T[k].flags, T + k, x, y, pc, A, rng);
+  precalc(xf->precalc, x, y, pc);
   float rx = 0.0f, ry = 0.0f;
   for (uint32_t k = npre; k < nt; ++k) {
-    float P[MAX_PRM];
-#pragma unroll
-    for (int j = 0; j < MAX_PRM; ++j) P[j] = T[k].prm[j];
-    const float w = T[k].weight;
-    float ox, oy;
-    eval_var(T[k].var, x, y, pc, w, A, P, rng, ox, oy);
-    if (!(T[k].flags & TF_NO_WEIGHT_MUL)) { ox = w * ox; oy = w * oy; }
-    if (k == npre) { rx = ox; ry = oy; }
-    else { rx = rx + ox; ry = ry + oy; }
+    reg_term(T[k].var, T[k].flags, T + k, x, y, pc, A, rng, k == npre, rx, ry);
     // Everything a variation computes from x, y, the precalcs or the affine is
     // invariant in this loop, and loop-invariant code motion would hoist it out
     // of the switch -- i.e. evaluate sin(x), sin(rsq), exp(x - 1), ... of all 76
@@ -366,19 +408,36 @@ __device__ __forceinline__ XOut apply_xform(BP blk, int xi, float x0, float y0,
                  "+v"(pc.cosa), "+v"(pc.sinr), "+v"(pc.cosr));
     asm volatile("" : "+s"(A[0]), "+s"(A[1]), "+s"(A[2]), "+s"(A[3]), "+s"(A[4]), "+s"(A[5]));
   }
-  if (xf->flags & XF_HAS_POST) {
-    const float q0 = xf->post[0], q1 = xf->post[1], q2 = xf->post[2];
-    const float q3 = xf->post[3], q4 = xf->post[4], q5 = xf->post[5];
-    float px = fmaf(q0, rx, fmaf(q2, ry, q4));
-    float py = fmaf(q1, rx, fmaf(q3, ry, q5));
-    rx = px; ry = py;
-  }
-  float cin = first ? rng.randf() : c0;
-  XOut o;
-  o.x = rx; o.y = ry;
-  o.c = rf_mix(cin, xf->color, xf->color_speed);
-  o.w = xf->opacity;
-  return o;
+  return finish_xform(xf, (xf->flags & XF_HAS_POST) != 0u, rx, ry, c0, first, rng);
+}
+
+// A list of term codes (variation id | TF_* flags << 16) fixed at compile time.
+template <uint32_t... C>
+struct TermList {};
+
+// Applies xform `xi` whose structure is fixed at compile time: the precalc
+// mask PCM, whether it has a post-affine, and its pre_xform and regular terms
+// in Block order.  Coefficients, weights and parameters are still read from
+// the Block, so a flame's values can change without a new program.
+template <uint32_t PCM, bool POST, class Rng, class BP, uint32_t... PRE, uint32_t... REG>
+__device__ __forceinline__ XOut apply_fixed(TermList<PRE...>, TermList<REG...>, BP blk, int xi, float x0,
+                                            float y0, float c0, bool first, Rng& rng) {
+  const auto* __restrict__ xf = &blk->xf[xi];
+  float A[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) A[k] = xf->aff[k];
+  float x = fmaf(A[0], x0, fmaf(A[2], y0, A[4]));
+  float y = fmaf(A[1], x0, fmaf(A[3], y0, A[5]));
+  const auto* __restrict__ T = blk->terms + xf->term_begin;
+  Pre pc;
+  pc.r = pc.rsq = pc.a = pc.phi = pc.sina = pc.cosa = pc.sinr = pc.cosr = 0.0f;
+  uint32_t k = 0;
+  ((pre_term(PRE & 0xffffu, PRE >> 16, T + k, x, y, pc, A, rng), ++k), ...);
+  precalc(PCM, x, y, pc);
+  float rx = 0.0f, ry = 0.0f;
+  bool ft = true;
+  ((reg_term(REG & 0xffffu, REG >> 16, T + k, x, y, pc, A, rng, ft, rx, ry), ++k, ft = false), ...);
+  return finish_xform(xf, POST, rx, ry, c0, first, rng);
 }
 
 // get_xform_id (shaders/templates/xform_select.tpl.glsl:1-14): the first i

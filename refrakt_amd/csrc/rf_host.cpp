@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <numeric>
 #include <sstream>
 #include <string>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "rf_host.hpp"
+#include "rf_jit.hpp"
 #include "rf_internal.hpp"
 
 #include "gen/variation_table.inc"
@@ -341,6 +343,7 @@ static int flame_upload(rf_flame* f, float tss_width) {
   const uint32_t T = ctx->sp.temporal_samples;
   f->animated = animated;
   f->n_blocks = animated ? T : 1u;
+  f->host_block.assign(1, b);
   HIPCHK(ctx, f->blocks.alloc(sizeof(Block) * (animated ? (size_t)T + 1 : 1)));
   const size_t pristine = animated ? T : 0;
   HIPCHK(ctx, hipMemcpyAsync((Block*)f->blocks.p + pristine, &b, sizeof(Block), hipMemcpyHostToDevice, ctx->stream));
@@ -393,6 +396,29 @@ static inline rfk::FastArgs fast_args(rf_ctx* ctx, rf_flame* f) {
   return a;
 }
 
+// Launches the fast engine over the flame's P particles with histogram mode
+// hm (1..5): its flame program when programs are on, else the precompiled
+// interpreter.  A flame under RF_HIST_AUTO that starts with the LDS table
+// gets the sort kernel compiled with it, the mode it usually moves to.
+static int fast_launch(rf_ctx* ctx, rf_flame* f, const rfk::FastArgs& a, uint64_t P, int hm) {
+  const int W = fast_waves(ctx);
+  if (ctx->programs && !f->host_block.empty()) {
+    std::vector<std::pair<int, int>> ks{{W, hm}};
+    if (ctx->sp.hist_mode == RF_HIST_AUTO && hm == RF_HIST_LDS) ks.push_back({W, RF_HIST_SORT});
+    if (ctx->sp.hist_mode == RF_HIST_AUTO && hm == RF_HIST_SORT) ks.push_back({W, RF_HIST_LDS});
+    std::string err;
+    const rfj::Program* p = rfj::get_program(f->host_block[0], ks, err);
+    if (!p) return set_err(ctx, RF_ERR_HIP, err);
+    f->program_compile_s = rfj::program_compile_seconds(p);
+    f->program_used = 1;
+    HIPCHK(ctx, rfj::launch_fast(p, a, P, W, hm, ctx->stream));
+    return RF_OK;
+  }
+  f->program_used = 0;
+  HIPCHK(ctx, rfk::launch_fast(a, P, W, hm, ctx->stream));
+  return RF_OK;
+}
+
 static int do_warmup(rf_ctx* ctx, rf_flame* f, uint32_t num_passes, float tss, const uint32_t* rows_in) {
   if (!ctx || !f || f->ctx != ctx) return set_err(ctx, RF_ERR_ARG, "bad context or flame");
   const uint64_t P = ctx->sp.total_particles;
@@ -431,7 +457,16 @@ static int do_warmup(rf_ctx* ctx, rf_flame* f, uint32_t num_passes, float tss, c
     a.n_iter = num_passes + 1;
     a.first_run = 1; a.do_draw = 0;
     a.bins = nullptr;
-    HIPCHK(ctx, rfk::launch_fast(a, P, fast_waves(ctx), ctx->sp.hist_mode, ctx->stream));
+    // No sample is binned, so any histogram mode's kernel gives the same
+    // particles; a flame program uses the kernel its first draw needs (the
+    // sort kernel, with its one-workgroup-per-batch grid, stands in for the
+    // LDS table's resident grid).
+    int hm = ctx->sp.hist_mode;
+    if (ctx->programs) {
+      hm = ctx->sp.hist_mode == RF_HIST_AUTO ? f->auto_mode : ctx->sp.hist_mode;
+      if (hm == RF_HIST_LDS && ctx->sp.hist_mode == RF_HIST_AUTO) hm = RF_HIST_SORT;
+    }
+    if ((rc = fast_launch(ctx, f, a, P, hm))) return rc;
   }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   f->have_particles = true;
@@ -622,7 +657,8 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     if (num_iter) {
       if (hm_run == RF_HIST_SORT) HIPCHK(ctx, rfk::launch_sort_reset(a.sort, ctx->stream));
-      HIPCHK(ctx, rfk::launch_fast(a, P, fast_waves(ctx), hm_run, ctx->stream));
+      const int rc3 = fast_launch(ctx, f, a, P, hm_run);
+      if (rc3) return rc3;
       if (hm_run == RF_HIST_SORT) {
         HIPCHK(ctx, rfk::launch_sort_rebucket(a.sort, a.bins, a.palette, ctx->stream));
         HIPCHK(ctx, rfk::launch_sort_tiles(a.sort, a.bins, h->w * h->h, a.palette, ctx->stream));
@@ -702,6 +738,7 @@ int rf_create(const rf_sim_params* p, rf_ctx** out) {
   rf_ctx* ctx = new rf_ctx();
   ctx->sp = *p;
   if (ctx->sp.shuffle_count == 0) ctx->sp.shuffle_count = 1024;
+  if (const char* env = getenv("RF_FLAME_PROGRAMS")) ctx->programs = strcmp(env, "0") != 0;
   if (p->block_waves != 0 && p->block_waves != 4 && p->block_waves != 8) {
     g_create_err = "block_waves must be 0 (default 4), 4 or 8";
     delete ctx;
@@ -855,6 +892,65 @@ int rf_rgba8_device(rf_ctx* ctx, const void* in, void* out, uint32_t w, uint32_t
 }
 
 double rf_flame_sort_overflow(const rf_flame* f) { return f ? f->last_sort_overflow : -1.0; }
+
+int rf_set_flame_programs(rf_ctx* ctx, int32_t enable) {
+  if (!ctx) return RF_ERR_ARG;
+  if (enable != 0 && enable != 1) return set_err(ctx, RF_ERR_ARG, "enable must be 0 or 1");
+  ctx->programs = enable == 1;
+  return RF_OK;
+}
+
+int rf_flame_program_info(const rf_flame* f, double* compile_seconds) {
+  if (!f) return RF_ERR_ARG;
+  if (compile_seconds) *compile_seconds = f->program_compile_s;
+  return f->program_used;
+}
+
+static int program_request(const rf_flame_desc* d, int32_t waves, int32_t hm, Block& b, std::string& err) {
+  if (!d) { err = "null description"; return RF_ERR_ARG; }
+  if (waves != 0 && waves != 4 && waves != 8) { err = "block_waves must be 0, 4 or 8"; return RF_ERR_ARG; }
+  if (hm < RF_HIST_QUAD || hm > RF_HIST_SORT) { err = "hist_mode must be 1..5"; return RF_ERR_ARG; }
+  if (!build_block(*d, b, err)) return RF_ERR_ARG;
+  return RF_OK;
+}
+
+int rf_flame_program_source(const rf_flame_desc* d, int32_t waves, int32_t hm, char* buf, size_t cap) {
+  std::unique_ptr<Block> b(new Block);
+  std::string err;
+  if (program_request(d, waves, hm, *b, err)) return RF_ERR_ARG;
+  const std::string src = rfj::program_source(*b, {{waves ? waves : 8, hm}});
+  if (buf && cap) {
+    const size_t n = std::min(cap - 1, src.size());
+    memcpy(buf, src.data(), n);
+    buf[n] = '\0';
+  }
+  return (int)src.size() + 1;
+}
+
+int rf_flame_program_build(const rf_flame_desc* d, int32_t waves, int32_t hm, const char* arch, void* code,
+                           size_t cap, size_t* size, char* log, size_t log_len, double* seconds) {
+  std::unique_ptr<Block> b(new Block);
+  std::string err;
+  auto put_log = [&](const std::string& m) {
+    if (log && log_len) {
+      const size_t n = std::min(log_len - 1, m.size());
+      memcpy(log, m.data(), n);
+      log[n] = '\0';
+    }
+  };
+  if (program_request(d, waves, hm, *b, err)) { put_log(err); return RF_ERR_ARG; }
+  if (!arch || !*arch) { put_log("arch is empty"); return RF_ERR_ARG; }
+  std::vector<char> obj;
+  std::string clog;
+  if (!rfj::compile(rfj::program_source(*b, {{waves ? waves : 8, hm}}), arch, obj, clog, seconds)) {
+    put_log(clog);
+    return RF_ERR_HIP;
+  }
+  put_log(clog);
+  if (size) *size = obj.size();
+  if (code && cap >= obj.size()) memcpy(code, obj.data(), obj.size());
+  return (code && cap < obj.size()) ? RF_ERR_ARG : RF_OK;
+}
 
 int rf_flame_hist_mode(const rf_flame* f, double* frac) {
   if (!f) return RF_ERR_ARG;

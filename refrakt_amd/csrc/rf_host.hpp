@@ -87,6 +87,7 @@ struct rf_ctx {
   rfh::DevBuf render_img;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_draw_ms = 0.0;   // device time of the last draw_to_bins' chaos-game launches
+  bool programs = true;        // fast engine: run flame programs (rf_jit.cpp)
 };
 
 struct rf_flame {
@@ -113,6 +114,11 @@ struct rf_flame {
   uint32_t sort_bins = 0;
   std::vector<double> sort_bfrac, sort_tfrac;
   double last_sort_overflow = -1.0;
+  // Flame program: the host copy of the Block (its structure selects the
+  // program), whether the last fast-engine launch ran one.
+  std::vector<rfd::Block> host_block;
+  int program_used = -1;
+  double program_compile_s = 0.0;
 };
 
 struct rf_hist {

@@ -1,8 +1,10 @@
 // Internal interface between the host library (rf_host.cpp) and the kernel
 // launchers (rf_k_compat.hip, rf_k_fast.hip, rf_k_misc.hip).
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "rf_device.hpp"
 #include "rf_sort.hpp"
@@ -63,6 +65,7 @@ struct DEArgs {
 
 constexpr int NPERM = 16;         // permutation rows of the fast engine (held in LDS)
 
+#ifndef __HIPCC_RTC__
 hipError_t launch_init_rs(uint4* rs, uint64_t n, hipStream_t s);
 hipError_t launch_compat_pass(const CompatArgs& a, int hist_mode, hipStream_t s);
 hipError_t launch_fast(const FastArgs& a, uint64_t n_particles, int waves, int hist_mode, hipStream_t s);
@@ -75,5 +78,6 @@ hipError_t launch_eval_xform(const rfd::Block* blk, const float* pts, const floa
                              uint32_t n, hipStream_t s);
 hipError_t launch_atomic_probe(float* bins, uint64_t nbins, uint64_t samples, int hist_mode,
                                const float4* palette, hipStream_t s);
+#endif
 
 }  // namespace rfk

@@ -1,0 +1,57 @@
+// Flame programs: the fast engine's kernel compiled per flame structure.
+//
+// The reference generates and compiles one GLSL dispatch() per flame
+// (src/variation_table.cpp:217-265, src/flame.cpp:285-288).  Here the
+// precompiled kernels interpret a flame from its Block; a flame program is
+// the same kernel body (rf_k_fast_body.hpp) instantiated with the flame's
+// structure -- which variations each xform runs, its precalcs, post-affine and
+// final xform -- fixed at compile time with hiprtc, so the per-term variation
+// switch, the term loop and their scalar loads disappear.  Coefficients,
+// weights and parameters stay in the Block: a flame whose values change but
+// whose structure does not keeps its program.  Programs are cached per
+// structure and device for the life of the process.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "rf_device.hpp"
+#include "rf_internal.hpp"
+
+namespace rfj {
+
+struct Program;
+
+// Text identifying a Block's structure (programs are shared between flames
+// with equal keys).
+std::string structure_key(const rfd::Block& b);
+
+// HIP source of the program for `b` with one entry point per (waves, hist
+// mode) pair in `kernels`.
+std::string program_source(const rfd::Block& b, const std::vector<std::pair<int, int>>& kernels);
+
+// Name of the entry point for (waves, hist mode).
+std::string entry_name(int waves, int hist_mode);
+
+// Compiles `src` for `arch` (e.g. "gfx950") with hiprtc; returns false and
+// sets log on failure.
+bool compile(const std::string& src, const std::string& arch, std::vector<char>& code, std::string& log,
+             double* seconds);
+
+// The program for b's structure on the current device with entry points for
+// every (waves, hist mode) in `kernels`, compiled and loaded on first use.
+// Returns nullptr and sets err on failure.
+const Program* get_program(const rfd::Block& b, const std::vector<std::pair<int, int>>& kernels,
+                           std::string& err);
+
+// Seconds spent compiling the program (0 if it came from the cache).
+double program_compile_seconds(const Program* p);
+
+// Launches the program's fast-engine kernel over n particles, like
+// rfk::launch_fast for the precompiled kernels.
+hipError_t launch_fast(const Program* p, const rfk::FastArgs& a, uint64_t n, int waves, int hist_mode,
+                       hipStream_t s);
+
+}  // namespace rfj

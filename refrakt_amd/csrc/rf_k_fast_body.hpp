@@ -1,0 +1,236 @@
+// Fast engine kernel body, shared by the precompiled kernels (rf_k_fast.hip,
+// flames interpreted from their Block) and by compiled flame programs
+// (rf_jit.cpp, the same body instantiated with a flame's structure fixed).
+#pragma once
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#endif
+
+#include "rf_kern_common.hpp"
+
+namespace rfk {
+namespace detail {
+using namespace rfd;
+
+// ------------------------------------------------------------ fast engine
+// Persistent chaos game.  A batch is 64*WAVES particles; a workgroup takes
+// batches blockIdx.x, blockIdx.x + gridDim.x, ... and runs each for all
+// n_iter iterations with the particles in registers.  Per iteration each wave
+// draws one xform from its own wave-uniform stream (so the variation switch
+// never diverges), applies it, bins, and then the batch is permuted through
+// LDS with one of NPERM random permutations chosen by a batch-uniform stream,
+// which decorrelates the xform sequences of particles sharing a wave (the
+// role of the reference's shuffle rows, src/shuffle_buffers.cpp).  All random
+// streams are keyed by particle, wave-of-batch and batch index, never by the
+// workgroup, so the results do not depend on the grid size.
+//
+// HM 1/2/3/5 launch one workgroup per batch; HM 4 (LDS bin cache) launches
+// one workgroup per resident slot so that each workgroup's LDS table spans
+// many batches before it is flushed.
+// HIST_SORT staging: the samples of up to SORT_STAGE_SLOTS / N iterations wait in
+// LDS (one fixed slot per lane and iteration, no append atomics).  A flush
+// ranks them by bucket (counting sort), reserves one run per non-empty bucket
+// with one global atomic, places the records in bucket order in LDS and
+// writes them out so that neighbouring lanes store neighbouring addresses.
+template <int N>
+constexpr int sort_stage_iters() { return SORT_STAGE_SLOTS / N; }
+
+template <int N, bool ON>
+struct SortStage {
+  static constexpr int S = SORT_STAGE_SLOTS;
+  uint32_t bin[S];
+  uint32_t rec[S];       // bucket-ordered records
+  uint16_t rank[S];
+  uint8_t pid[S];
+  uint8_t bkt[S];        // bucket of rec[j]
+  uint32_t cnt[SORT_MAX_BUCKETS];
+  uint32_t off[SORT_MAX_BUCKETS];
+  uint32_t gb[SORT_MAX_BUCKETS];
+  uint32_t boff[SORT_MAX_BUCKETS];   // bucket offsets and capacities, staged once per workgroup
+  uint32_t bcap[SORT_MAX_BUCKETS];
+  uint32_t total;
+};
+template <int N>
+struct SortStage<N, false> {};
+
+template <int N>
+__device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_slots, const SortArgs& a,
+                                           float* __restrict__ bins, const float4* __restrict__ pal) {
+  static_assert(SORT_MAX_BUCKETS <= 4 * WAVE, "scan width");
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t b = tid; b < a.n_buckets; b += N) st.cnt[b] = 0u;
+  lds_barrier();
+  for (uint32_t i = tid; i < n_slots; i += N) {
+    const uint32_t bin = st.bin[i];
+    if (bin != SORT_NONE) st.rank[i] = (uint16_t)atomicAdd(&st.cnt[bin >> SORT_COARSE_SHIFT], 1u);
+  }
+  lds_barrier();
+  wave0_exclusive_scan<4>(st.cnt, st.off, a.n_buckets, &st.total);
+  for (uint32_t b = tid; b < a.n_buckets; b += N) {
+    const uint32_t c = st.cnt[b];
+    st.gb[b] = c ? atomicAdd(a.bcur + b, c) : 0u;
+  }
+  lds_barrier();
+  for (uint32_t i = tid; i < n_slots; i += N) {
+    const uint32_t bin = st.bin[i];
+    if (bin != SORT_NONE) {
+      const uint32_t b = bin >> SORT_COARSE_SHIFT;
+      const uint32_t j = st.off[b] + st.rank[i];
+      st.rec[j] = ((bin & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | st.pid[i];
+      st.bkt[j] = (uint8_t)b;
+    }
+  }
+  lds_barrier();
+  const uint32_t total = st.total;
+  const uint32_t rounds = (total + N - 1) / N;  // uniform
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t j = r * N + tid;
+    bool ovf = false;
+    uint32_t bin = 0, pidx = 0;
+    if (j < total) {
+      const uint32_t b = st.bkt[j];
+      const uint32_t rec = st.rec[j];
+      const uint32_t dst = st.gb[b] + (j - st.off[b]);
+      if (dst < st.bcap[b]) {
+        a.cbuf[st.boff[b] + dst] = rec;
+      } else {  // over the bucket's capacity: straight to the histogram
+        ovf = true;
+        bin = (b << SORT_COARSE_SHIFT) | (rec >> 8);
+        pidx = rec & 255u;
+      }
+    }
+    bin_quad(bins, bin, ovf, pidx, 1.0f, pal);
+  }
+  lds_barrier();
+}
+
+template <int WAVES, int HM, int LOGK, class Prog>
+__device__ __forceinline__ void k_fast_body(const FastArgs& g, const Block* __restrict__ blocks) {
+  constexpr int N = 64 * WAVES;
+  // Double-buffered exchange: iteration `it` writes buffer it&1, so one
+  // barrier per iteration separates every write from the reads it must not
+  // overtake (the next write to the same buffer is two barriers later).
+  __shared__ float s_x[2][N], s_y[2][N], s_c[2][N];
+  __shared__ float4 s_pal[256];
+  // The permutation rows live in LDS: a global load inside the loop would sit
+  // behind the iteration's atomics in the in-order vmcnt queue and stall.
+  __shared__ uint16_t s_perm[NPERM * N];
+  __shared__ LdsBins<LOGK> s_bins;
+  __shared__ SortStage<N, HM == 5> s_sort;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palette[i];
+  for (uint32_t i = tid; i < (uint32_t)(NPERM * N); i += N) s_perm[i] = g.perms[i];
+  if constexpr (HM == 4) {
+    lds_bins_clear(s_bins);
+    lds_bins_palette(s_bins, g.palette);
+  }
+  if constexpr (HM == 5) {
+    if (g.do_draw) {
+      for (uint32_t b = tid; b < g.sort.n_buckets; b += N) {
+        s_sort.boff[b] = g.sort.boff[b];
+        s_sort.bcap[b] = g.sort.bcap[b];
+      }
+    }
+  }
+  __syncthreads();
+
+  unsigned long long drawn = 0;
+  uint32_t hits = 0;
+  for (uint32_t batch = blockIdx.x; batch < g.n_batches; batch += gridDim.x) {
+    const uint32_t base = batch * (uint32_t)N;
+    const uint32_t pid = base + tid;
+    // Integer division runs on the VALU, so the quotient is not known to be
+    // uniform; without readfirstlane every Block access would be a vector load.
+    const uint32_t t = __builtin_amdgcn_readfirstlane(base / g.ppt);
+    CBlock* blk = as_constant(blocks + (size_t)t * g.block_stride);
+
+    Jsf32 lrng = jsf_from_philox(g.seed, pid, g.launch, 0u);
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(batch * WAVES + (tid >> 6));
+    Jsf32 wrng = jsf_from_philox(g.seed, wid, g.launch, 1u);
+    Jsf32 grng = jsf_from_philox(g.seed, batch, g.launch, 2u);
+
+    float x, y, c;
+    if (g.first_run) {
+      // Hammersley point (src/hammersley.cpp:29-48) of index pid mod ppt, jittered
+      // like the first run of shaders/flame.glsl:63-66.
+      const uint32_t cnt = g.ppt;
+      const uint32_t mx = (cnt % 2u == 0u) ? cnt : (1u << (32 - __clz(cnt - 1u)));
+      const float inv_max = rf_div_rn(1.0f, (float)mx);
+      const uint32_t rbits = 31u - __clz(mx);
+      const uint32_t i = pid % cnt;
+      const uint32_t fl = __brev(i) >> (32u - rbits);
+      x = ((float)i * inv_max) * 2.0f - 1.0f;
+      y = ((float)fl * inv_max) * 2.0f - 1.0f;
+      c = 0.0f;
+      const float r1 = lrng.randf();
+      const float r2 = lrng.randf();
+      const float s = ((r1 * 0.1f) * PI) * 2.0f;
+      float sn, cs;
+      rf_sincos01(rf_sqrt_rn(r2), &sn, &cs);
+      x = x + s * sn;
+      y = y + s * cs;
+    } else {
+      const float4 p = g.pos[pid];
+      x = p.x; y = p.y; c = p.z;
+    }
+
+    for (uint32_t it = 0; it < g.n_iter; ++it) {
+      // The draw is wave-uniform; selecting on the raw integer keeps xid in SGPRs
+      // (float compares would run on the VALU, make xid look divergent and turn
+      // every xform/term access into vector loads behind the atomics).
+      const int xid = Prog::select(blk, wrng.next_a());
+      XOut o = Prog::apply(blk, xid, x, y, c, g.first_run && it == 0, lrng);
+      if (rf_badval(o.x) || rf_badval(o.y)) {
+        // A particle that left the finite plane restarts at a random point and
+        // is not drawn this iteration (the reference leaves it NaN forever,
+        // shaders/flame.glsl:70 is commented out).
+        o.x = lrng.randf() * 2.0f - 1.0f;
+        o.y = lrng.randf() * 2.0f - 1.0f;
+        o.w = 0.0f;
+      }
+      x = o.x; y = o.y; c = o.c;
+      if (g.do_draw) {
+        const Located L = locate<Prog>(blk, o, lrng, g.ss, g.W, g.H);
+        if constexpr (HM == 4) {
+          bin_lds(s_bins, g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal, hits);
+        } else if constexpr (HM == 5) {
+          const uint32_t slot = (it % sort_stage_iters<N>()) * N + tid;
+          s_sort.bin[slot] = L.valid ? L.bidx : SORT_NONE;
+          s_sort.pid[slot] = (uint8_t)L.pidx;
+        } else {
+          bin_sample<HM>(g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal);
+        }
+        drawn += (unsigned long long)__popcll(__ballot(L.valid));
+      }
+      const uint32_t row = grng.next_a() & (uint32_t)(NPERM - 1);
+      const uint32_t dst = s_perm[row * N + tid];
+      const uint32_t b = it & 1u;
+      s_x[b][dst] = x; s_y[b][dst] = y; s_c[b][dst] = c;
+      lds_barrier();
+      x = s_x[b][tid]; y = s_y[b][tid]; c = s_c[b][tid];
+      if constexpr (HM == 5) {
+        // the exchange barrier above also orders the staging writes before the flush
+        constexpr uint32_t SI = sort_stage_iters<N>();
+        if (g.do_draw && ((it % SI) == SI - 1 || it + 1 == g.n_iter))
+          sort_flush(s_sort, ((it % SI) + 1) * (uint32_t)N, g.sort, g.bins, s_pal);
+      }
+    }
+    g.pos[pid] = make_float4(x, y, c, 0.0f);
+    // The next batch's first exchange writes buffer 0, which slower waves may
+    // still be reading from this batch's last iteration.
+    lds_barrier();
+  }
+  if (g.do_draw) {
+    count_drawn(g.counter, drawn);
+    if constexpr (HM == 4) {
+      __syncthreads();
+      flush_lds(s_bins, g.bins);
+      const unsigned long long wh = (unsigned long long)wave_sum_u32(hits);
+      if ((threadIdx.x & (WAVE - 1)) == 0 && wh) atomicAdd(g.counter + 1, wh);
+    }
+  }
+}
+
+}  // namespace detail
+}  // namespace rfk

@@ -1,7 +1,9 @@
 // Binning and drawing helpers shared by the chaos-game kernels.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "rf_internal.hpp"
 
@@ -283,12 +285,31 @@ struct Located {
   bool valid;
 };
 
-template <class Rng, class BP>
+// How a chaos-game kernel applies a flame's xforms.  InterpProg reads the
+// structure (variations, precalcs, post-affine, final xform) from the Block at
+// run time; a compiled flame program (rf_jit.cpp) supplies the same four
+// functions with the structure fixed at compile time.
+struct InterpProg {
+  template <class BP>
+  static __device__ __forceinline__ int select(BP blk, uint32_t raw) { return select_xform_raw(blk, raw); }
+  template <class Rng, class BP>
+  static __device__ __forceinline__ XOut apply(BP blk, int xi, float x, float y, float c, bool first, Rng& rng) {
+    return apply_xform(blk, xi, x, y, c, first, rng);
+  }
+  template <class BP>
+  static __device__ __forceinline__ bool has_final(BP blk) { return blk->has_final != 0u; }
+  template <class Rng, class BP>
+  static __device__ __forceinline__ XOut apply_final(BP blk, float x, float y, float c, Rng& rng) {
+    return apply_xform(blk, FINAL, x, y, c, false, rng);
+  }
+};
+
+template <class Prog = InterpProg, class Rng, class BP>
 __device__ __forceinline__ Located locate(BP blk, const XOut& o, Rng& rng, const float ss[6], uint32_t W,
                                           uint32_t H) {
   float rx = o.x, ry = o.y, rc = o.c, rw = o.w;
-  if (blk->has_final) {
-    const XOut f = apply_xform(blk, FINAL, o.x, o.y, o.c, false, rng);
+  if (Prog::has_final(blk)) {
+    const XOut f = Prog::apply_final(blk, o.x, o.y, o.c, rng);
     rx = f.x; ry = f.y; rc = f.c; rw = f.w * o.w;
   }
   const float sx = fmaf(ss[0], rx, fmaf(ss[2], ry, ss[4]));

@@ -7,8 +7,10 @@
 // touched bin.  Records carry no weight, so the mode needs unit opacities;
 // the LDS sums are fixed point, so palette values must lie in [0, 1].
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace rfk {
 
@@ -37,8 +39,10 @@ struct SortArgs {
   uint32_t n_chunks, n_segs;
 };
 
+#ifndef __HIPCC_RTC__
 hipError_t launch_sort_reset(const SortArgs& a, hipStream_t s);
 hipError_t launch_sort_rebucket(const SortArgs& a, float* bins, const float4* palette, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, float* bins, uint32_t n_bins, const float4* palette, hipStream_t s);
+#endif
 
 }  // namespace rfk
