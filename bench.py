@@ -306,7 +306,8 @@ def main():
                    "binned_samples": drawn, "attempted_samples": attempted, "hist_mode": args.hist_mode,
                    "hist_mode_used": mode_used, "lds_hit_fraction": round(lds_hits, 4) if lds_hits >= 0 else None,
                    "block_waves": args.waves or 8, "parallelism": "single GPU",
-                   "flame_program": {"used": bool(prog_used == 1), "compile_s": round(prog_compile_s, 3)}},
+                   "flame_program": {"used": bool(prog_used == 1), "compile_s": round(prog_compile_s, 3)},
+                   "sort_overflow": round(fh.sort_overflow(), 6) if mode_used == 5 else None},
         "roofline": roofline,
         "de_tonemap_ms": round(de_ms, 3),
     }
