@@ -108,27 +108,21 @@ __host__ __device__ __forceinline__ void rf_sincos01(float q, float* s, float* c
 // compiler if-converts it, so every call paid for both paths; in a kernel with
 // ~100 trig call sites that multiplied the iteration's VALU work ~8x.
 //
-// sin/cos: Cody-Waite reduction by pi/2 with a three-part constant, then the
-// minimax polynomials of the Cephes library's sinf/cosf (S. L. Moshier) on
-// [-pi/4, pi/4] -- the same method Mesa's llvmpipe uses for GLSL sin/cos.
-// Accurate to a few ulp for |x| < ~1e5 (the j*C1 product is exact below 2^16).
+// sin/cos: Cody-Waite reduction by 2 pi with a three-part constant (the
+// j * 6.28125 product is exact for |j| < 2^16, so the reduced argument is
+// accurate to a few ulp for |x| < ~4e5), then the hardware v_sin_f32 /
+// v_cos_f32 on the argument in revolutions, [-1/2, 1/2]: max abs error
+// 3.5e-7 on [-pi, pi] (measured against double on MI355X), within the 2^-11
+// GLSL allows for sin/cos.  Seven instructions for both values against about
+// 25 for a polynomial pair with quadrant logic.
 __device__ __forceinline__ void rf_sincos(float x, float* s, float* c) {
-  const float j = rintf(x * 0.636619772367581343f);
-  float r = fmaf(-j, 1.5703125f, x);
-  r = fmaf(-j, 4.837512969970703125e-4f, r);
-  r = fmaf(-j, 7.54978995489188216e-8f, r);
-  const float z = r * r;
-  float ps = fmaf(fmaf(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f);
-  ps = fmaf(ps * z, r, r);
-  float pc = fmaf(fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z, 4.166664568298827e-2f);
-  pc = fmaf(pc * z, z, fmaf(-0.5f, z, 1.0f));
-  const int q = (int)j;
-  float sn = (q & 1) ? pc : ps;
-  float cs = (q & 1) ? ps : pc;
-  if (q & 2) sn = -sn;
-  if ((q + 1) & 2) cs = -cs;
-  *s = sn;
-  *c = cs;
+  const float j = rintf(x * 0.159154936671257019f);
+  float r = fmaf(-j, 6.28125f, x);
+  r = fmaf(-j, 1.93530716933310032e-3f, r);
+  r = fmaf(-j, 1.02531316770182457e-11f, r);
+  const float t = r * 0.159154936671257019f;
+  *s = __builtin_amdgcn_sinf(t);
+  *c = __builtin_amdgcn_cosf(t);
 }
 __device__ __forceinline__ float rf_sin(float x) { float s, c; rf_sincos(x, &s, &c); return s; }
 __device__ __forceinline__ float rf_cos(float x) { float s, c; rf_sincos(x, &s, &c); return c; }
