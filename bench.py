@@ -165,7 +165,7 @@ def bench_batch(rf, args):
     import torch
     from refrakt_amd import configs
     frames = configs.cfg4()
-    ctx = rf.Context(1 << 12, 1, 16, seed=12345)
+    ctx = rf.Context(1 << 12, 1, 16, seed=12345, flame_programs=not args.no_programs)
     b = ctx.batch([(f.flame, f.width, f.height, f.samples) for f in frames])
     t0 = time.perf_counter()
     cfg, tune_ms = b.tune(1.0 / 16)
@@ -193,7 +193,8 @@ def bench_batch(rf, args):
         "config": {"workload": f"cfg4: {n} frames, xforms 2-12, sizes 512^2..2048^2, 64-512 spp, one graph replay "
                                "per step", "binned_samples_per_step": drawn,
                    "requested_samples_per_step": int(sum(f.samples for f in frames)),
-                   "tuned": cfg, "tune_s": round(tune_s, 2), "parallelism": "single GPU"},
+                   "tuned": cfg, "tune_s": round(tune_s, 2), "parallelism": "single GPU",
+                   "flame_program": {"used": b.program_info()[0] == 1, "compile_s": round(b.program_info()[1], 3)}},
         "binned_samples_per_s": drawn * args.steps / wall,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "first": float(lat.min()), "last": float(lat.max()),

@@ -280,6 +280,11 @@ int rf_batch_get_config(const rf_batch* batch, rf_batch_config* cfg);
  * a copy of the batch whose sample counts are scaled by sample_fraction
  * (0 < f <= 1), then sets the fastest.  best/best_ms may be NULL. */
 int rf_batch_tune(rf_batch* batch, double sample_fraction, rf_batch_config* best, double* best_ms);
+/* 1 if the batch's chaos-game launch runs a flame program (every frame's
+ * xform i has the structure of the same xform of the frame with the most
+ * xforms, and flame programs are on), 0 if it is interpreted;
+ * *compile_seconds (may be NULL) = seconds spent compiling it. */
+int rf_batch_program_info(const rf_batch* batch, double* compile_seconds);
 /* Replays the graph: zero, chaos game, DE + tonemap of every frame; *ms is
  * the host wall time of the replay. */
 int rf_batch_run(rf_batch* batch, double* ms);
@@ -379,7 +384,8 @@ int rf_set_flame_programs(rf_ctx* ctx, int32_t enable);
  * spent compiling its program in this process (0 when it came from the cache). */
 int rf_flame_program_info(const rf_flame* flame, double* compile_seconds);
 /* HIP source of the flame program of desc with one entry point for
- * (block_waves, hist_mode) (hist_mode 1..5).  Writes at most cap bytes
+ * (block_waves, hist_mode): hist_mode 1..5 for the fast engine, 16 + 1 or
+ * 16 + 4 for the batched renderer's kernel.  Writes at most cap bytes
  * (NUL-terminated) and returns the full length + 1, or < 0 on error. */
 int rf_flame_program_source(const rf_flame_desc* desc, int32_t block_waves, int32_t hist_mode, char* buf,
                             size_t cap);

@@ -165,6 +165,7 @@ _SIGS = {
     "rf_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),
     "rf_screen_affine": (C.c_int, [C.POINTER(FlameDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     "rf_set_flame_programs": (C.c_int, [_P, C.c_int32]),
+    "rf_batch_program_info": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "rf_flame_program_info": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "rf_flame_program_source": (C.c_int, [C.POINTER(FlameDesc), C.c_int32, C.c_int32, C.c_char_p, C.c_size_t]),
     "rf_flame_program_build": (C.c_int, [C.POINTER(FlameDesc), C.c_int32, C.c_int32, C.c_char_p, C.c_void_p,
@@ -742,6 +743,12 @@ class Batch:
         ms = C.c_double()
         self.ctx.check(self.ctx.L.rf_batch_run(self.h, C.byref(ms)))
         return ms.value
+
+    def program_info(self):
+        """(1 if the chaos-game launch runs a flame program, else 0; compile seconds)."""
+        sec = C.c_double(0.0)
+        used = self.ctx.L.rf_batch_program_info(self.h, C.byref(sec))
+        return used, sec.value
 
     def frame_stats(self) -> dict:
         n = len(self.sizes)

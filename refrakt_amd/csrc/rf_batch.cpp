@@ -9,6 +9,7 @@
 #include "rf_batch.hpp"
 #include "rf_host.hpp"
 #include "rf_internal.hpp"
+#include "rf_jit.hpp"
 
 using rfd::Block;
 using rfh::DevBuf;
@@ -34,6 +35,11 @@ struct rf_batch {
   uint32_t launch = 0;             // RNG stream key of this batch (same for every shape)
   bool lds_ok = true;              // every frame allows the fixed-point LDS path
   bool ran = false;
+  // Flame program: the frame with the most xforms, when its structure covers
+  // every frame's (xform i runs the same variations in every frame).
+  std::vector<Block> prog_block;
+  int program_used = -1;
+  double program_compile_s = 0.0;
 
   ~rf_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -107,13 +113,25 @@ int capture(rf_batch* b) {
 
   if (b->exec) { HIPCHK(ctx, hipGraphExecDestroy(b->exec)); b->exec = nullptr; }
   if (b->graph) { HIPCHK(ctx, hipGraphDestroy(b->graph)); b->graph = nullptr; }
-  // Size the persistent grid outside the capture (occupancy queries).
-  HIPCHK(ctx, rfk::batch_prepare(c.block_waves, (c.hist_mode == RF_HIST_LDS && !b->lds_ok) ? RF_HIST_QUAD : c.hist_mode));
+  // Size the persistent grid and compile the flame program (if any) outside
+  // the capture.
+  const int hm = (c.hist_mode == RF_HIST_LDS && !b->lds_ok) ? RF_HIST_QUAD : c.hist_mode;
+  const rfj::Program* prog = nullptr;
+  if (ctx->programs && !b->prog_block.empty()) {
+    std::string err;
+    prog = rfj::get_program(b->prog_block[0], {{c.block_waves, rfj::kBatchKernel + hm}}, err);
+    if (!prog) return rfh::set_err(ctx, RF_ERR_HIP, err);
+    b->program_compile_s = rfj::program_compile_seconds(prog);
+  } else {
+    HIPCHK(ctx, rfk::batch_prepare(c.block_waves, hm));
+  }
+  b->program_used = prog ? 1 : 0;
   HIPCHK(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipError_t e = rfk::launch_batch_init(a, (float4*)b->hist.p, b->total_bins, (const uint32_t*)b->items.p,
                                         (uint32_t*)b->tiles_done.p, (unsigned long long*)b->t0.p, b->stream);
-  const int hm = (c.hist_mode == RF_HIST_LDS && !b->lds_ok) ? RF_HIST_QUAD : c.hist_mode;
-  if (e == hipSuccess) e = rfk::launch_batch(a, c.block_waves, hm, 0, b->stream);
+  if (e == hipSuccess)
+    e = prog ? rfj::launch_batch(prog, a, c.block_waves, hm, 0, b->stream)
+             : rfk::launch_batch(a, c.block_waves, hm, 0, b->stream);
   for (uint32_t f = 0; f < b->n && e == hipSuccess; ++f) {
     const rf_flame_desc& d = b->desc[f];
     rfk::DEArgs g{};
@@ -217,6 +235,11 @@ int rf_batch_create(rf_ctx* ctx, const rf_frame_desc* frames, uint32_t n, rf_bat
   }
   b->clock_khz = (double)khz;
   b->launch = 0x40000000u + ctx->launch++;
+  uint32_t big = 0;
+  for (uint32_t f = 1; f < n; ++f) if (blk[f].n_xforms > blk[big].n_xforms) big = f;
+  bool covers = true;
+  for (uint32_t f = 0; f < n && covers; ++f) covers = rfj::structure_covers(blk[big], blk[f]);
+  if (covers) b->prog_block.assign(1, blk[big]);
   const int rc = capture(b);
   if (rc != RF_OK) { delete b; return rc; }
   *out = b;
@@ -276,6 +299,12 @@ int rf_batch_tune(rf_batch* b, double fraction, rf_batch_config* best_out, doubl
   if (best_out) *best_out = best;
   if (best_ms_out) *best_ms_out = best_ms;
   return RF_OK;
+}
+
+int rf_batch_program_info(const rf_batch* b, double* compile_seconds) {
+  if (!b) return RF_ERR_ARG;
+  if (compile_seconds) *compile_seconds = b->program_compile_s;
+  return b->program_used;
 }
 
 int rf_batch_run(rf_batch* b, double* ms) {

@@ -1,7 +1,9 @@
 // Batched rendering: device-side frame table and kernel arguments.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "rf_device.hpp"
 
@@ -31,10 +33,12 @@ struct BatchArgs {
   uint32_t fuse, n_iter;          // unbinned and binned iterations per particle
 };
 
+#ifndef __HIPCC_RTC__
 // Caches the resident-workgroup count of a kernel variant (call before a capture).
 hipError_t batch_prepare(int waves, int hist_mode);
 hipError_t launch_batch(const BatchArgs& a, int waves, int hist_mode, uint32_t max_grid, hipStream_t s);
 hipError_t launch_batch_init(const BatchArgs& a, float4* hist, uint64_t n_bins, const uint32_t* items,
                              uint32_t* tiles_done, unsigned long long* t0, hipStream_t s);
+#endif
 
 }  // namespace rfk

@@ -18,7 +18,21 @@ namespace rfj {
 namespace {
 
 // The LDS bin-cache table size per workgroup, as the precompiled kernels use.
-int lds_logk(int hm) { return hm == 4 ? 10 : 0; }
+int lds_logk(int waves, int mode) {
+  if (mode >= kBatchKernel) return mode - kBatchKernel == 4 ? (waves == 8 ? 11 : 10) : 0;
+  return mode == 4 ? 10 : 0;
+}
+
+std::string xform_key(const rfd::Block& b, int xi) {
+  const rfd::Xform& X = b.xf[xi];
+  std::ostringstream o;
+  o << X.precalc << "," << (X.flags & rfd::XF_HAS_POST) << "," << X.n_pre;
+  for (uint32_t k = 0; k < X.n_terms; ++k) {
+    const rfd::Term& T = b.terms[X.term_begin + k];
+    o << "," << (T.var | (T.flags << 16));
+  }
+  return o.str();
+}
 
 void xform_args(std::ostringstream& o, const rfd::Block& b, int xi) {
   const rfd::Xform& X = b.xf[xi];
@@ -40,22 +54,21 @@ void xform_args(std::ostringstream& o, const rfd::Block& b, int xi) {
 std::string structure_key(const rfd::Block& b) {
   std::ostringstream o;
   o << b.n_xforms << (b.has_final ? "F" : "") << ":";
-  auto one = [&](int xi) {
-    const rfd::Xform& X = b.xf[xi];
-    o << X.precalc << "," << (X.flags & rfd::XF_HAS_POST) << "," << X.n_pre;
-    for (uint32_t k = 0; k < X.n_terms; ++k) {
-      const rfd::Term& T = b.terms[X.term_begin + k];
-      o << "," << (T.var | (T.flags << 16));
-    }
-    o << ";";
-  };
-  for (uint32_t i = 0; i < b.n_xforms; ++i) one((int)i);
-  if (b.has_final) one(rfd::FINAL);
+  for (uint32_t i = 0; i < b.n_xforms; ++i) o << xform_key(b, (int)i) << ";";
+  if (b.has_final) o << xform_key(b, rfd::FINAL) << ";";
   return o.str();
 }
 
-std::string entry_name(int waves, int hist_mode) {
-  return "rf_prog_w" + std::to_string(waves) + "_h" + std::to_string(hist_mode);
+bool structure_covers(const rfd::Block& big, const rfd::Block& b) {
+  if (b.n_xforms > big.n_xforms || (b.has_final != 0) != (big.has_final != 0)) return false;
+  for (uint32_t i = 0; i < b.n_xforms; ++i)
+    if (xform_key(b, (int)i) != xform_key(big, (int)i)) return false;
+  return !b.has_final || xform_key(b, rfd::FINAL) == xform_key(big, rfd::FINAL);
+}
+
+std::string entry_name(int waves, int mode) {
+  if (mode >= kBatchKernel) return "rf_batch_w" + std::to_string(waves) + "_h" + std::to_string(mode - kBatchKernel);
+  return "rf_prog_w" + std::to_string(waves) + "_h" + std::to_string(mode);
 }
 
 std::string program_source(const rfd::Block& b, const std::vector<std::pair<int, int>>& kernels) {
@@ -63,7 +76,7 @@ std::string program_source(const rfd::Block& b, const std::vector<std::pair<int,
   o << "// flame program: " << structure_key(b) << "\n"
     << "typedef unsigned char uint8_t;\ntypedef unsigned short uint16_t;\ntypedef unsigned int uint32_t;\n"
     << "typedef unsigned long uint64_t;\ntypedef int int32_t;\ntypedef long int64_t;\n"
-    << "#include \"rf_k_fast_body.hpp\"\n"
+    << "#include \"rf_k_fast_body.hpp\"\n#include \"rf_k_batch_body.hpp\"\n"
     << "namespace rfprog {\nusing namespace rfd;\n"
     << "struct Prog {\n"
     << "  template <class BP>\n"
@@ -93,11 +106,18 @@ std::string program_source(const rfd::Block& b, const std::vector<std::pair<int,
   }
   o << "  }\n};\n}  // namespace rfprog\n";
   for (const auto& k : kernels) {
-    const int W = k.first, hm = k.second;
-    o << "extern \"C\" __global__ __launch_bounds__(" << 64 * W << ") void " << entry_name(W, hm)
-      << "(rfk::FastArgs g, const rfd::Block* __restrict__ blocks) {\n"
-      << "  rfk::detail::k_fast_body<" << W << ", " << hm << ", " << lds_logk(hm) << ", rfprog::Prog>(g, blocks);\n"
-      << "}\n";
+    const int W = k.first, mode = k.second;
+    o << "extern \"C\" __global__ __launch_bounds__(" << 64 * W << ") void " << entry_name(W, mode);
+    if (mode >= kBatchKernel) {
+      o << "(rfk::BatchArgs g, const rfd::Block* __restrict__ blocks,\n"
+        << "    const rfk::BatchFrame* __restrict__ frames) {\n"
+        << "  rfk::detail::k_batch_body<" << W << ", " << mode - kBatchKernel << ", " << lds_logk(W, mode)
+        << ", rfprog::Prog>(g, blocks, frames);\n}\n";
+    } else {
+      o << "(rfk::FastArgs g, const rfd::Block* __restrict__ blocks) {\n"
+        << "  rfk::detail::k_fast_body<" << W << ", " << mode << ", " << lds_logk(W, mode)
+        << ", rfprog::Prog>(g, blocks);\n}\n";
+    }
   }
   return o.str();
 }
@@ -147,7 +167,7 @@ struct Program {
   int device = -1;
   std::vector<hipModule_t> modules;
   std::map<std::pair<int, int>, hipFunction_t> fns;
-  std::map<std::pair<int, int>, int> resident;  // HIST_LDS: resident workgroups (grid size)
+  std::map<std::pair<int, int>, int> resident;  // persistent kernels: resident workgroups (grid size)
   double compile_s = 0.0;
 };
 
@@ -198,7 +218,7 @@ const Program* get_program(const rfd::Block& b, const std::vector<std::pair<int,
     hipFunction_t fn = nullptr;
     e = hipModuleGetFunction(&fn, mod, entry_name(k.first, k.second).c_str());
     if (e != hipSuccess) { err = std::string("hipModuleGetFunction: ") + hipGetErrorString(e); return nullptr; }
-    if (k.second == 4) {
+    if (k.second == 4 || k.second >= kBatchKernel) {
       int per_cu = 0, cus = 0;
       e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * k.first, 0);
       if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -211,6 +231,21 @@ const Program* get_program(const rfd::Block& b, const std::vector<std::pair<int,
 }
 
 double program_compile_seconds(const Program* p) { return p ? p->compile_s : 0.0; }
+
+hipError_t launch_batch(const Program* p, const rfk::BatchArgs& a0, int waves, int hist_mode, uint32_t max_grid,
+                        hipStream_t s) {
+  const std::pair<int, int> k{waves, kBatchKernel + hist_mode};
+  const auto it = p->fns.find(k);
+  if (it == p->fns.end()) return hipErrorNotFound;
+  uint32_t grid = std::min<uint32_t>(a0.n_items, (uint32_t)p->resident.at(k));
+  if (max_grid) grid = std::min(grid, max_grid);
+  if (grid == 0) return hipSuccess;
+  rfk::BatchArgs a = a0;
+  const rfd::Block* blocks = a.blocks;
+  const rfk::BatchFrame* frames = a.frames;
+  void* args[] = {&a, &blocks, &frames};
+  return hipModuleLaunchKernel(it->second, grid, 1, 1, 64u * (uint32_t)waves, 1, 1, 0, s, args, nullptr);
+}
 
 hipError_t launch_fast(const Program* p, const rfk::FastArgs& a0, uint64_t n, int waves, int hist_mode,
                        hipStream_t s) {

@@ -18,11 +18,17 @@
 #include <vector>
 
 #include "rf_device.hpp"
+#include "rf_batch.hpp"
 #include "rf_internal.hpp"
 
 namespace rfj {
 
 struct Program;
+
+// Kernels of a program are (waves, mode) pairs: mode 1..5 is the fast
+// engine with that histogram mode, kBatchKernel + mode the batched renderer
+// (modes 1 and 4).
+constexpr int kBatchKernel = 16;
 
 // Text identifying a Block's structure (programs are shared between flames
 // with equal keys).
@@ -53,5 +59,14 @@ double program_compile_seconds(const Program* p);
 // rfk::launch_fast for the precompiled kernels.
 hipError_t launch_fast(const Program* p, const rfk::FastArgs& a, uint64_t n, int waves, int hist_mode,
                        hipStream_t s);
+
+// Launches the program's batched-rendering kernel (one workgroup per
+// resident slot, at most max_grid if non-zero), like rfk::launch_batch.
+hipError_t launch_batch(const Program* p, const rfk::BatchArgs& a, int waves, int hist_mode, uint32_t max_grid,
+                        hipStream_t s);
+
+// True when every xform of `b` has the structure of the same xform of `big`
+// and both have the same final xform: a program for `big` runs `b`.
+bool structure_covers(const rfd::Block& big, const rfd::Block& b);
 
 }  // namespace rfj

@@ -118,3 +118,39 @@ def test_batch_autotune_picks_a_candidate(rf, gpu_available):
     b.run()
     assert b.frame_stats()["drawn"].min() > 0
     b.close()
+
+
+@pytest.mark.parametrize("hist_mode", [1, 4])
+def test_batch_program_matches_interpreter(rf, gpu_available, hist_mode):
+    """Every frame's xform k has the same variations, so the batch runs one
+    flame program for all frames; it computes the interpreter's bits: drawn
+    counts and hit counts equal, colour sums within 1e-5 (atomic order)."""
+    frames = _frames(rf)
+    out = []
+    for programs in (True, False):
+        ctx = rf.Context(1 << 12, 1, 16, seed=5, flame_programs=programs)
+        b = ctx.batch(frames)
+        b.set_config(8, hist_mode, 64, 20)
+        b.run()
+        out.append((b.program_info()[0], b.frame_stats()["drawn"].copy(),
+                    [b.histogram(i) for i in range(len(frames))]))
+        b.close()
+        ctx.close()
+    (ua, da, ha), (ub, db, hb) = out
+    assert ua == 1 and ub == 0
+    assert np.array_equal(da, db)
+    for a, c in zip(ha, hb):
+        assert np.array_equal(a[..., 3], c[..., 3])
+        np.testing.assert_allclose(a[..., :3], c[..., :3], rtol=1e-5, atol=1e-5)
+
+
+def test_batch_with_mixed_structures_is_interpreted(rf, gpu_available):
+    frames = _frames(rf)
+    fl = frames[1][0]
+    fl.xforms[0].variations = {"linear": 0.3, "heart": 0.7}  # xform 0 differs from the other frames'
+    ctx = rf.Context(1 << 12, 1, 16, seed=5)
+    b = ctx.batch(frames)
+    b.set_config(8, rf.RF_HIST_LDS, 64, 20)
+    b.run()
+    assert b.program_info()[0] == 0
+    assert (b.frame_stats()["drawn"] > 0).all()

@@ -133,3 +133,15 @@ def test_program_cfg2_flame_matches_interpreter(rf, gpu_available):
     assert a[3] == b[3]
     assert np.array_equal(a[2][..., 3], b[2][..., 3])
     np.testing.assert_allclose(a[2][..., :3], b[2][..., :3], rtol=1e-5, atol=1e-5)
+
+
+def test_batch_program_compiles(rf):
+    """The batched renderer's kernel body compiles as a program for the
+    family structure of the cfg4 frames (mode 16 + 4: LDS table)."""
+    from refrakt_amd import configs
+    frames = configs.cfg4()
+    big = max((f.flame for f in frames), key=lambda fl: len(fl.xforms))
+    src = rf.program_source(big, 8, 16 + 4)
+    assert "rf_batch_w8_h4" in src and "k_batch_body<8, 4, 11, rfprog::Prog>" in src
+    code, log, sec = rf.program_build(big, 8, 16 + 4, "gfx950")
+    assert len(code) > 4096, log
