@@ -79,6 +79,131 @@ __global__ __launch_bounds__(256) void k_conv3x3(View in, const bf16* __restrict
   }
 }
 
+// LDS-tiled implicit GEMM.  A workgroup of 4 waves covers CT_R = 4 image rows
+// x CT_W = 64 pixels (one row per wave, four 16-pixel A fragments) and 16*NT
+// output channels.  Per 16-channel step of Cin it stages the (CT_R + 2) x
+// (CT_W + 2) input patch (zero halo) and the step's 3x3 x 16*NT x 16 weights
+// in LDS, 48 bytes per pixel / weight row (16 channels + pad: 8-byte fragment
+// reads of 16 consecutive rows hit distinct banks, 16-byte stores stay
+// aligned); every A fragment then feeds NT MFMAs and every B fragment four.
+// The next step's operands are loaded into registers while the MFMAs of the
+// current one run.  Needs 16-byte aligned channel runs (Cin, channel
+// strides and offsets multiples of 8).
+constexpr int CT_W = 64, CT_R = 4, CT_PS = 24;
+constexpr int CT_IN_ROWS = (CT_R + 2) * (CT_W + 2);  // staged input pixels
+
+template <int NT>
+struct ConvStage {
+  static constexpr int IN_Q = CT_IN_ROWS * 2;   // 16-byte pieces of the input patch
+  static constexpr int W_Q = 9 * 16 * NT * 2;   // 16-byte pieces of the weights
+  static constexpr int IN_PER = (IN_Q + 255) / 256;
+  static constexpr int W_PER = (W_Q + 255) / 256;
+  uint4 vin[IN_PER];
+  uint4 vw[W_PER];
+};
+
+template <int NT>
+__device__ __forceinline__ void conv_load(ConvStage<NT>& st, const View& in, const bf16* __restrict__ w, int n,
+                                          int y0, int x0, int co0, int c0, int H, int W, int Cin, int Cout) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < ConvStage<NT>::IN_PER; ++k) {
+    const int q = tid + 256 * k;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (q < ConvStage<NT>::IN_Q) {
+      const int half = q & 1, pix = q >> 1;
+      const int yy = y0 + pix / (CT_W + 2) - 1, gx = x0 + pix % (CT_W + 2) - 1;
+      if (yy >= 0 && yy < H && gx >= 0 && gx < W)
+        v = *reinterpret_cast<const uint4*>(in.p + ((size_t)(n * H + yy) * W + gx) * in.cs + in.co + c0 + 8 * half);
+    }
+    st.vin[k] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < ConvStage<NT>::W_PER; ++k) {
+    const int q = tid + 256 * k;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (q < ConvStage<NT>::W_Q) {
+      const int half = q & 1, row = q >> 1;  // row = tap * 16 * NT + output channel
+      const int tap = row / (16 * NT), co = row % (16 * NT);
+      v = *reinterpret_cast<const uint4*>(w + ((size_t)tap * Cout + co0 + co) * Cin + c0 + 8 * half);
+    }
+    st.vw[k] = v;
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __restrict__ w,
+                                                     const float* __restrict__ bias, View out, int H, int W,
+                                                     int Cin, int Cout, int relu) {
+  __shared__ __attribute__((aligned(16))) bf16 s_in[CT_IN_ROWS * CT_PS];
+  __shared__ __attribute__((aligned(16))) bf16 s_w[9 * 16 * NT * CT_PS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nrb = (H + CT_R - 1) / CT_R;
+  const int n = blockIdx.y / nrb, y0 = (blockIdx.y % nrb) * CT_R;
+  const int x0 = blockIdx.x * CT_W;
+  const int co0 = blockIdx.z * 16 * NT;
+  const int i = lane & 15, kq = lane >> 4;
+  float4_t acc[4][NT];
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[f][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+  ConvStage<NT> st;
+  conv_load<NT>(st, in, w, n, y0, x0, co0, 0, H, W, Cin, Cout);
+  for (int c0 = 0; c0 < Cin; c0 += 16) {
+    __syncthreads();  // the previous step's fragment reads are done
+#pragma unroll
+    for (int k = 0; k < ConvStage<NT>::IN_PER; ++k) {
+      const int q = tid + 256 * k;
+      if (q < ConvStage<NT>::IN_Q)
+        *reinterpret_cast<uint4*>(s_in + (q >> 1) * CT_PS + 8 * (q & 1)) = st.vin[k];
+    }
+#pragma unroll
+    for (int k = 0; k < ConvStage<NT>::W_PER; ++k) {
+      const int q = tid + 256 * k;
+      if (q < ConvStage<NT>::W_Q) *reinterpret_cast<uint4*>(s_w + (q >> 1) * CT_PS + 8 * (q & 1)) = st.vw[k];
+    }
+    __syncthreads();
+    if (c0 + 16 < Cin) conv_load<NT>(st, in, w, n, y0, x0, co0, c0 + 16, H, W, Cin, Cout);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        short4_t b[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          b[t] = *reinterpret_cast<const short4_t*>(s_w + ((r * 3 + s) * 16 * NT + 16 * t + i) * CT_PS + 4 * kq);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          const short4_t a = *reinterpret_cast<const short4_t*>(
+              s_in + ((wave + r) * (CT_W + 2) + 16 * f + i + s) * CT_PS + 4 * kq);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[f][t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  const int y = y0 + wave;
+  if (y >= H) return;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = co0 + 16 * t + i;
+    const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int px = x0 + 16 * f + 4 * kq + e;
+        if (px < W) {
+          float v = acc[f][t][e] + bv;
+          if (relu) v = fmaxf(v, 0.0f);
+          out.p[((size_t)(n * H + y) * W + px) * out.cs + out.co + co] = f2bf(v);
+        }
+      }
+    }
+  }
+}
+
 // Weight gradient on the VALU: a workgroup owns a 16 (out) x 16 (in) channel
 // block, one thread per (k, c) pair with the 9 taps in registers, and walks
 // image rows; each 64-pixel segment of dy and the three input rows around it
@@ -195,6 +320,52 @@ __global__ void k_upsample2(View in, View out, int N, int H, int W, int C) {
   out.p[po * out.cs + out.co + c] = in.p[((size_t)(n * H + yo / 2) * W + xo / 2) * in.cs + in.co + c];
 }
 
+// 8-channel (16-byte) versions of pooling and upsampling: one thread per
+// output pixel and channel group.
+__device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
+
+__global__ void k_avgpool2_v8(View in, View out, int N, int H, int W, int C) {
+  const int Ho = H / 2, Wo = W / 2, G = C / 8;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N * Ho * Wo * G) return;
+  const int g = (int)(i % G);
+  const size_t po = i / G;
+  const int xo = (int)(po % Wo), yo = (int)((po / Wo) % Ho), n = (int)(po / ((size_t)Wo * Ho));
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const uint4 v = *reinterpret_cast<const uint4*>(
+          in.p + ((size_t)(n * H + 2 * yo + a) * W + 2 * xo + b) * in.cs + in.co + 8 * g);
+      const uint32_t q[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) { s[2 * k] += bf_lo(q[k]); s[2 * k + 1] += bf_hi(q[k]); }
+    }
+  uint32_t r[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    r[k] = (uint32_t)f2bf(0.25f * s[2 * k]) | ((uint32_t)f2bf(0.25f * s[2 * k + 1]) << 16);
+  *reinterpret_cast<uint4*>(out.p + po * out.cs + out.co + 8 * g) = make_uint4(r[0], r[1], r[2], r[3]);
+}
+
+__global__ void k_upsample2_v8(View in, View out, int N, int H, int W, int C) {
+  const int Ho = 2 * H, Wo = 2 * W, G = C / 8;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N * Ho * Wo * G) return;
+  const int g = (int)(i % G);
+  const size_t po = i / G;
+  const int xo = (int)(po % Wo), yo = (int)((po / Wo) % Ho), n = (int)(po / ((size_t)Wo * Ho));
+  *reinterpret_cast<uint4*>(out.p + po * out.cs + out.co + 8 * g) =
+      *reinterpret_cast<const uint4*>(in.p + ((size_t)(n * H + yo / 2) * W + xo / 2) * in.cs + in.co + 8 * g);
+}
+
+inline bool v8_ok(const View& a, const View& b, int C) {
+  return C % 8 == 0 && a.cs % 8 == 0 && a.co % 8 == 0 && b.cs % 8 == 0 && b.co % 8 == 0 &&
+         ((uintptr_t)a.p & 15u) == 0 && ((uintptr_t)b.p & 15u) == 0;
+}
+
 __global__ void k_upsample2_bwd(View dout, View din, int N, int H, int W, int C) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)N * H * W * C) return;
@@ -285,6 +456,16 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
   if (Cin % 16 || Cout % 16 || in.cs % 4 || in.co % 4 || out.cs < out.co + Cout || in.cs < in.co + Cin)
     return hipErrorInvalidValue;
   const int nt = (Cout % 64 == 0) ? 4 : (Cout % 32 == 0) ? 2 : 1;
+  if (Cin % 8 == 0 && in.cs % 8 == 0 && in.co % 8 == 0 && ((uintptr_t)in.p & 15u) == 0 &&
+      ((uintptr_t)w & 15u) == 0) {
+    const dim3 grid((W + CT_W - 1) / CT_W, N * ((H + CT_R - 1) / CT_R), Cout / (16 * nt));
+    switch (nt) {
+      case 4: hipLaunchKernelGGL(k_conv3x3_lds<4>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
+      case 2: hipLaunchKernelGGL(k_conv3x3_lds<2>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
+      default: hipLaunchKernelGGL(k_conv3x3_lds<1>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
+    }
+    return hipGetLastError();
+  }
   const dim3 grid((W + 63) / 64, N * H, Cout / (16 * nt));
   switch (nt) {
     case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
@@ -314,7 +495,11 @@ hipError_t relu_mask(View dy, View y, int npix, int C, hipStream_t s) {
 }
 
 hipError_t avgpool2(View in, View out, int N, int H, int W, int C, hipStream_t s) {
-  hipLaunchKernelGGL(k_avgpool2, grid1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s, in, out, N, H, W, C);
+  if (v8_ok(in, out, C))
+    hipLaunchKernelGGL(k_avgpool2_v8, grid1((size_t)N * (H / 2) * (W / 2) * (C / 8)), dim3(256), 0, s, in, out, N, H,
+                       W, C);
+  else
+    hipLaunchKernelGGL(k_avgpool2, grid1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s, in, out, N, H, W, C);
   return hipGetLastError();
 }
 
@@ -324,7 +509,10 @@ hipError_t avgpool2_bwd(View dout, View din, int N, int H, int W, int C, int acc
 }
 
 hipError_t upsample2(View in, View out, int N, int H, int W, int C, hipStream_t s) {
-  hipLaunchKernelGGL(k_upsample2, grid1((size_t)N * 4 * H * W * C), dim3(256), 0, s, in, out, N, H, W, C);
+  if (v8_ok(in, out, C))
+    hipLaunchKernelGGL(k_upsample2_v8, grid1((size_t)N * 4 * H * W * (C / 8)), dim3(256), 0, s, in, out, N, H, W, C);
+  else
+    hipLaunchKernelGGL(k_upsample2, grid1((size_t)N * 4 * H * W * C), dim3(256), 0, s, in, out, N, H, W, C);
   return hipGetLastError();
 }
 
