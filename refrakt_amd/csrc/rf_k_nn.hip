@@ -81,62 +81,86 @@ __global__ __launch_bounds__(256) void k_conv3x3(View in, const bf16* __restrict
 
 // LDS-tiled implicit GEMM.  A workgroup of 4 waves covers CT_R = 4 image rows
 // x CT_W = 64 pixels (one row per wave, four 16-pixel A fragments) and 16*NT
-// output channels.  Per 16-channel step of Cin it stages the (CT_R + 2) x
-// (CT_W + 2) input patch (zero halo) and the step's 3x3 x 16*NT x 16 weights
-// in LDS, 48 bytes per pixel / weight row (16 channels + pad: 8-byte fragment
-// reads of 16 consecutive rows hit distinct banks, 16-byte stores stay
-// aligned); every A fragment then feeds NT MFMAs and every B fragment four.
-// The next step's operands are loaded into registers while the MFMAs of the
-// current one run.  Needs 16-byte aligned channel runs (Cin, channel
-// strides and offsets multiples of 8).
-constexpr int CT_W = 64, CT_R = 4, CT_PS = 24;
+// output channels.  Per KC-channel step of Cin (KC = 32 with gfx950's
+// v_mfma_f32_16x16x32_bf16 when Cin allows, else 16 with the 16x16x16 form)
+// it stages the (CT_R + 2) x (CT_W + 2) input patch (zero halo) and the
+// step's 3x3 x 16*NT x KC weights in LDS, KC + 8 channels per pixel / weight
+// row (the pad spreads 16 consecutive rows over distinct banks and keeps
+// 16-byte stores aligned); every A fragment then feeds NT MFMAs and every B
+// fragment four.  The next step's operands are loaded into registers while
+// the MFMAs of the current one run.  Needs 16-byte aligned channel runs (Cin,
+// channel strides and offsets multiples of 8).
+// Operand layouts (wave64, i = lane % 16, kq = lane / 16): 16x16x16 lane
+// holds A[i][4 kq + e], B[4 kq + e][i] (e < 4); 16x16x32 lane holds
+// A[i][8 kq + e], B[8 kq + e][i] (e < 8); both give D[4 kq + e][i] (e < 4).
+constexpr int CT_W = 64, CT_R = 4;
 constexpr int CT_IN_ROWS = (CT_R + 2) * (CT_W + 2);  // staged input pixels
 
-template <int NT>
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+template <int NT, int KC>
 struct ConvStage {
-  static constexpr int IN_Q = CT_IN_ROWS * 2;   // 16-byte pieces of the input patch
-  static constexpr int W_Q = 9 * 16 * NT * 2;   // 16-byte pieces of the weights
+  static constexpr int PS = KC + 8;             // LDS row stride (bf16)
+  static constexpr int PQ = KC / 8;             // 16-byte pieces per pixel / weight row
+  static constexpr int IN_Q = CT_IN_ROWS * PQ;
+  static constexpr int W_Q = 9 * 16 * NT * PQ;
   static constexpr int IN_PER = (IN_Q + 255) / 256;
   static constexpr int W_PER = (W_Q + 255) / 256;
   uint4 vin[IN_PER];
   uint4 vw[W_PER];
 };
 
-template <int NT>
-__device__ __forceinline__ void conv_load(ConvStage<NT>& st, const View& in, const bf16* __restrict__ w, int n,
+template <int NT, int KC>
+__device__ __forceinline__ void conv_load(ConvStage<NT, KC>& st, const View& in, const bf16* __restrict__ w, int n,
                                           int y0, int x0, int co0, int c0, int H, int W, int Cin, int Cout) {
+  using S = ConvStage<NT, KC>;
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < ConvStage<NT>::IN_PER; ++k) {
+  for (int k = 0; k < S::IN_PER; ++k) {
     const int q = tid + 256 * k;
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if (q < ConvStage<NT>::IN_Q) {
-      const int half = q & 1, pix = q >> 1;
+    if (q < S::IN_Q) {
+      const int piece = q % S::PQ, pix = q / S::PQ;
       const int yy = y0 + pix / (CT_W + 2) - 1, gx = x0 + pix % (CT_W + 2) - 1;
       if (yy >= 0 && yy < H && gx >= 0 && gx < W)
-        v = *reinterpret_cast<const uint4*>(in.p + ((size_t)(n * H + yy) * W + gx) * in.cs + in.co + c0 + 8 * half);
+        v = *reinterpret_cast<const uint4*>(in.p + ((size_t)(n * H + yy) * W + gx) * in.cs + in.co + c0 + 8 * piece);
     }
     st.vin[k] = v;
   }
 #pragma unroll
-  for (int k = 0; k < ConvStage<NT>::W_PER; ++k) {
+  for (int k = 0; k < S::W_PER; ++k) {
     const int q = tid + 256 * k;
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if (q < ConvStage<NT>::W_Q) {
-      const int half = q & 1, row = q >> 1;  // row = tap * 16 * NT + output channel
+    if (q < S::W_Q) {
+      const int piece = q % S::PQ, row = q / S::PQ;  // row = tap * 16 * NT + output channel
       const int tap = row / (16 * NT), co = row % (16 * NT);
-      v = *reinterpret_cast<const uint4*>(w + ((size_t)tap * Cout + co0 + co) * Cin + c0 + 8 * half);
+      v = *reinterpret_cast<const uint4*>(w + ((size_t)tap * Cout + co0 + co) * Cin + c0 + 8 * piece);
     }
     st.vw[k] = v;
   }
 }
 
-template <int NT>
+template <int KC>
+__device__ __forceinline__ float4_t conv_mfma(const bf16* ap, const bf16* bp, float4_t acc);
+template <>
+__device__ __forceinline__ float4_t conv_mfma<16>(const bf16* ap, const bf16* bp, float4_t acc) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const short4_t*>(ap),
+                                                   *reinterpret_cast<const short4_t*>(bp), acc, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ float4_t conv_mfma<32>(const bf16* ap, const bf16* bp, float4_t acc) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8_t*>(ap),
+                                                 *reinterpret_cast<const bf16x8_t*>(bp), acc, 0, 0, 0);
+}
+
+template <int NT, int KC>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __restrict__ w,
                                                      const float* __restrict__ bias, View out, int H, int W,
                                                      int Cin, int Cout, int relu) {
-  __shared__ __attribute__((aligned(16))) bf16 s_in[CT_IN_ROWS * CT_PS];
-  __shared__ __attribute__((aligned(16))) bf16 s_w[9 * 16 * NT * CT_PS];
+  using S = ConvStage<NT, KC>;
+  constexpr int PS = S::PS, KE = KC / 4;  // fragment elements per lane
+  __shared__ __attribute__((aligned(16))) bf16 s_in[CT_IN_ROWS * PS];
+  __shared__ __attribute__((aligned(16))) bf16 s_w[9 * 16 * NT * PS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nrb = (H + CT_R - 1) / CT_R;
   const int n = blockIdx.y / nrb, y0 = (blockIdx.y % nrb) * CT_R;
@@ -148,38 +172,32 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __rest
   for (int f = 0; f < 4; ++f)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[f][t] = float4_t{0.f, 0.f, 0.f, 0.f};
-  ConvStage<NT> st;
-  conv_load<NT>(st, in, w, n, y0, x0, co0, 0, H, W, Cin, Cout);
-  for (int c0 = 0; c0 < Cin; c0 += 16) {
+  S st;
+  conv_load<NT, KC>(st, in, w, n, y0, x0, co0, 0, H, W, Cin, Cout);
+  for (int c0 = 0; c0 < Cin; c0 += KC) {
     __syncthreads();  // the previous step's fragment reads are done
 #pragma unroll
-    for (int k = 0; k < ConvStage<NT>::IN_PER; ++k) {
+    for (int k = 0; k < S::IN_PER; ++k) {
       const int q = tid + 256 * k;
-      if (q < ConvStage<NT>::IN_Q)
-        *reinterpret_cast<uint4*>(s_in + (q >> 1) * CT_PS + 8 * (q & 1)) = st.vin[k];
+      if (q < S::IN_Q) *reinterpret_cast<uint4*>(s_in + (q / S::PQ) * PS + 8 * (q % S::PQ)) = st.vin[k];
     }
 #pragma unroll
-    for (int k = 0; k < ConvStage<NT>::W_PER; ++k) {
+    for (int k = 0; k < S::W_PER; ++k) {
       const int q = tid + 256 * k;
-      if (q < ConvStage<NT>::W_Q) *reinterpret_cast<uint4*>(s_w + (q >> 1) * CT_PS + 8 * (q & 1)) = st.vw[k];
+      if (q < S::W_Q) *reinterpret_cast<uint4*>(s_w + (q / S::PQ) * PS + 8 * (q % S::PQ)) = st.vw[k];
     }
     __syncthreads();
-    if (c0 + 16 < Cin) conv_load<NT>(st, in, w, n, y0, x0, co0, c0 + 16, H, W, Cin, Cout);
+    if (c0 + KC < Cin) conv_load<NT, KC>(st, in, w, n, y0, x0, co0, c0 + KC, H, W, Cin, Cout);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
       for (int s = 0; s < 3; ++s) {
-        short4_t b[NT];
+        const bf16* bp = s_w + ((r * 3 + s) * 16 * NT + i) * PS + KE * kq;
+        const bf16* ap = s_in + ((wave + r) * (CT_W + 2) + i + s) * PS + KE * kq;
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
-          b[t] = *reinterpret_cast<const short4_t*>(s_w + ((r * 3 + s) * 16 * NT + 16 * t + i) * CT_PS + 4 * kq);
+        for (int f = 0; f < 4; ++f)
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          const short4_t a = *reinterpret_cast<const short4_t*>(
-              s_in + ((wave + r) * (CT_W + 2) + 16 * f + i + s) * CT_PS + 4 * kq);
-#pragma unroll
-          for (int t = 0; t < NT; ++t) acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[f][t], 0, 0, 0);
-        }
+          for (int t = 0; t < NT; ++t) acc[f][t] = conv_mfma<KC>(ap + 16 * f * PS, bp + 16 * t * PS, acc[f][t]);
       }
     }
   }
@@ -459,11 +477,22 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
   if (Cin % 8 == 0 && in.cs % 8 == 0 && in.co % 8 == 0 && ((uintptr_t)in.p & 15u) == 0 &&
       ((uintptr_t)w & 15u) == 0) {
     const dim3 grid((W + CT_W - 1) / CT_W, N * ((H + CT_R - 1) / CT_R), Cout / (16 * nt));
-    switch (nt) {
-      case 4: hipLaunchKernelGGL(k_conv3x3_lds<4>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
-      case 2: hipLaunchKernelGGL(k_conv3x3_lds<2>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
-      default: hipLaunchKernelGGL(k_conv3x3_lds<1>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
+#define RF_CONV_LDS(NT_, KC_) \
+  hipLaunchKernelGGL((k_conv3x3_lds<NT_, KC_>), grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu)
+    // K = 32 steps pay off where each staged step feeds 64 output channels;
+    // the narrow full-resolution layers keep the smaller K = 16 tiles, whose
+    // LDS and registers fit four workgroups per CU (measured: 32-channel
+    // layers 1.15 ms with K = 16 against 1.53 ms with K = 32 at 7680x4320).
+    if (Cin % 32 == 0 && nt == 4) {
+      RF_CONV_LDS(4, 32);
+    } else {
+      switch (nt) {
+        case 4: RF_CONV_LDS(4, 16); break;
+        case 2: RF_CONV_LDS(2, 16); break;
+        default: RF_CONV_LDS(1, 16); break;
+      }
     }
+#undef RF_CONV_LDS
     return hipGetLastError();
   }
   const dim3 grid((W + 63) / 64, N * H, Cout / (16 * nt));
