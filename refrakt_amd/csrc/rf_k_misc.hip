@@ -133,36 +133,72 @@ __device__ __forceinline__ void de_stamp(const DEArgs& g) {
   }
 }
 
+// Splat weights of every radius 0..R, computed once per workgroup with
+// de_weight's arithmetic: entry (i + r) * (2r + 1) + (j + r) of radius r
+// starts at de_wt_off(r) = sum over q < r of (2q + 1)^2.
+__device__ __forceinline__ int de_wt_off(int r) { return (4 * r * r * r - r) / 3; }
+
 __global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
-  extern __shared__ unsigned char smem[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int R = g.R;
   const int SW = DE_TW + 2 * R, SH = DE_TH + 2 * R;
   float4* s_col = reinterpret_cast<float4*>(smem);
-  DESrc* s_m = reinterpret_cast<DESrc*>(smem + sizeof(float4) * SW * SH);
+  int* s_r = reinterpret_cast<int*>(smem + sizeof(float4) * SW * SH);  // source radius, -1: empty
+  float* s_wt = reinterpret_cast<float*>(s_r + SW * SH);
+  __shared__ int s_rmax;
   const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * DE_TH;
   const int tid = threadIdx.x;
+  if (tid == 0) s_rmax = 0;
+  for (int r = 0; r <= R; ++r) {
+    const int S = 2 * r + 1;
+    DESrc m;
+    m.r = (float)r;
+    m.e = rf_div_rn(1.0f, (float)S);
+    m.nrm = (r > 0) ? rf_div_rn(0.63661977236f, (float)(r * r)) : 1.0f;
+    for (int q = tid; q < S * S; q += DE_TW * DE_TH) s_wt[de_wt_off(r) + q] = de_weight(m, q / S - r, q % S - r);
+  }
+  __syncthreads();
+  int rloc = 0;
   for (int k = tid; k < SW * SH; k += DE_TW * DE_TH) {
     const int ly = k / SW, lx = k % SW;
     float4 col; DESrc m;
     de_source(g, tx0 - R + lx, ty0 - R + ly, col, m);
     s_col[k] = col;
-    s_m[k] = m;
+    s_r[k] = (int)m.r;
+    rloc = max(rloc, (int)m.r);
   }
+  // A source contributes only within its own radius (de_weight is 0 beyond
+  // it), so offsets beyond the largest radius staged for this tile add
+  // nothing: the loops stop there, and per offset only sources whose radius
+  // reaches it are read.  The sum is unchanged bit for bit (zero-weight terms
+  // were skipped before too).
+  rloc = max(rloc, __shfl_xor(rloc, 32));
+  rloc = max(rloc, __shfl_xor(rloc, 16));
+  rloc = max(rloc, __shfl_xor(rloc, 8));
+  rloc = max(rloc, __shfl_xor(rloc, 4));
+  rloc = max(rloc, __shfl_xor(rloc, 2));
+  rloc = max(rloc, __shfl_xor(rloc, 1));
+  if ((tid & 63) == 0) atomicMax(&s_rmax, rloc);
   __syncthreads();
+  const int RM = min(R, s_rmax);
   const int lx = tid % DE_TW, ly = tid / DE_TW;
   const int tx = tx0 + lx, ty = ty0 + ly;
   if (tx < (int)g.W && ty < (int)g.H) {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int i = -R; i <= R; ++i) {
+    for (int i = -RM; i <= RM; ++i) {
       // source row cy = ty - i -> LDS row ly + R - i
       const int row = (ly + R - i) * SW;
-      for (int j = -R; j <= R; ++j) {
+      const int ai = i < 0 ? -i : i;
+      for (int j = -RM; j <= RM; ++j) {
         const int k = row + lx + R - j;
-        const DESrc m = s_m[k];
-        const float wgt = de_weight(m, i, j);
-        if (wgt != 0.0f) {
-          const float4 c = s_col[k];
-          acc.x += c.x * wgt; acc.y += c.y * wgt; acc.z += c.z * wgt; acc.w += c.w * wgt;
+        const int r = s_r[k];
+        const int aj = j < 0 ? -j : j;
+        if (r >= ai && r >= aj) {
+          const float wgt = s_wt[de_wt_off(r) + (i + r) * (2 * r + 1) + (j + r)];
+          if (wgt != 0.0f) {
+            const float4 c = s_col[k];
+            acc.x += c.x * wgt; acc.y += c.y * wgt; acc.z += c.z * wgt; acc.w += c.w * wgt;
+          }
         }
       }
     }
@@ -258,8 +294,9 @@ hipError_t launch_animate(Block* blocks, uint32_t T, float tss_width, const floa
 
 hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s) {
   const int SW = DE_TW + 2 * a.R, SH = DE_TH + 2 * a.R;
-  const size_t sm = (size_t)SW * SH * (sizeof(float4) + sizeof(DESrc));
-  if (sm <= 65536) {  // within the default dynamic-LDS limit of one launch (R <= 13)
+  const size_t n_wt = (size_t)(4 * (a.R + 1) * (a.R + 1) * (a.R + 1) - (a.R + 1)) / 3;  // de_wt_off(R + 1)
+  const size_t sm = (size_t)SW * SH * (sizeof(float4) + sizeof(int)) + n_wt * sizeof(float);
+  if (sm <= 65536) {  // within the default dynamic-LDS limit of one launch (R <= 14)
     dim3 grid((a.W + DE_TW - 1) / DE_TW, (a.H + DE_TH - 1) / DE_TH);
     hipLaunchKernelGGL(k_de_tiled, grid, dim3(DE_TW * DE_TH), sm, s, a);
   } else {
