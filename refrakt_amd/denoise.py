@@ -11,6 +11,7 @@ target.  The network, its kernels and Adam live in the library
 (rf_denoiser_*); PyTorch only holds the device tensors of the data set.
 
     python -m refrakt_amd.denoise --flames 2000 --steps 3000 --out weights.npz
+    python -m refrakt_amd.denoise --time 4320x7680,1024x1024    (inference timing)
 """
 from __future__ import annotations
 
@@ -152,8 +153,34 @@ def train(n_flames: int = 2000, steps: int = 3000, batch: int = 8, crop: int = 2
     return dn, meta
 
 
+def time_inference(sizes, reps: int = 3, seed: int = 1):
+    """Host wall time of one inference per (H, W) in `sizes` (after one
+    warm-up call), on random bf16 inputs with the committed weights."""
+    import torch
+    ctx = Context(1 << 12, 1, 16)
+    dn = ctx.denoiser(seed)
+    load_weights(dn)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    out = []
+    for H, W in sizes:
+        x = (torch.rand((1, H, W, 16), device="cuda", generator=g) * 2).to(torch.bfloat16)
+        y = torch.empty((1, H, W, 4), dtype=torch.float32, device="cuda")
+        dn.infer(x.data_ptr(), y.data_ptr(), 1, H, W)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dn.infer(x.data_ptr(), y.data_ptr(), 1, H, W)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        out.append({"event": "infer", "height": H, "width": W, "ms": round(ms, 3)})
+        print(json.dumps(out[-1]), flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--time", default=None,
+                    help="time inference instead of training, e.g. 4320x7680,1024x1024 (height x width)")
     ap.add_argument("--flames", type=int, default=2000)
     ap.add_argument("--val-flames", type=int, default=64)
     ap.add_argument("--steps", type=int, default=3000)
@@ -163,6 +190,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
+    if a.time:
+        time_inference([tuple(int(v) for v in s.split("x")) for s in a.time.split(",")])
+        return
     train(a.flames, a.steps, a.batch, a.crop, a.lr, a.seed, a.out, val_flames=a.val_flames)
 
 
