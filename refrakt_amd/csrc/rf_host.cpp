@@ -531,12 +531,17 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
       jobs.push_back((b << 16) | k);
     }
   const uint32_t n_chunks = (uint32_t)jobs.size();
-  for (uint32_t t = 0; t < nt_used; ++t)
+  std::vector<uint32_t> packed;
+  for (uint32_t t = 0; t < nt_used; ++t) {
+    if (tcap[t] <= (uint32_t)rfk::SORT_SEG_PACKED) { packed.push_back(t); continue; }
     for (uint32_t k = 0; k * (uint64_t)rfk::SORT_SEG < tcap[t]; ++k) {
       if (k > 0xffffu) return set_err(ctx, RF_ERR_LIMIT, "sort histogram: tile too large");
       jobs.push_back((t << 16) | k);
     }
+  }
   const uint32_t n_segs = (uint32_t)jobs.size() - n_chunks;
+  const uint32_t n_psegs = (uint32_t)packed.size();
+  jobs.insert(jobs.end(), packed.begin(), packed.end());
   meta.insert(meta.end(), jobs.begin(), jobs.end());
   if (ctx->sort_cbuf.bytes < cb * 4) HIPCHK(ctx, ctx->sort_cbuf.alloc(cb * 4));
   if (ctx->sort_tbuf.bytes < ct * 4) HIPCHK(ctx, ctx->sort_tbuf.alloc(ct * 4));
@@ -556,6 +561,8 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
   a.tcap = dm + 2 * (size_t)nb + nt;
   a.chunks = dm + meta_fixed;
   a.segs = dm + meta_fixed + n_chunks;
+  a.psegs = dm + meta_fixed + n_chunks + n_segs;
+  a.n_psegs = n_psegs;
   a.n_buckets = nb;
   a.n_tiles = nt;
   a.n_chunks = n_chunks;

@@ -202,6 +202,83 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(SortArgs a, float4*
   }
 }
 
+// Tiles of at most SORT_SEG_PACKED records: one workgroup per tile, two
+// 64-bit LDS atomics per record (see SORT_SEG_PACKED), then one plain
+// read-modify-write per touched bin.  36 KiB of LDS: four workgroups per CU.
+struct TileSumsPacked {
+  uint64_t rg[TILE_BINS];   // red | green << 32, 2^16 per unit
+  uint64_t bc[TILE_BINS];   // blue | count << 32
+  uint64_t prg[256];        // per palette entry: red | green << 32
+  uint64_t pb[256];         // blue | 1 << 32
+};
+
+__device__ __forceinline__ uint32_t fix16(float v) { return (uint32_t)(v * 65536.0f + 0.5f); }
+
+__device__ __forceinline__ void tile_add_packed(TileSumsPacked& t, uint32_t r) {
+  const uint32_t p = r & 255u, i = r >> 8;
+  lds_add_u64(&t.rg[i], t.prg[p]);
+  lds_add_u64(&t.bc[i], t.pb[p]);
+}
+
+__global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles_packed(SortArgs a, float4* __restrict__ hist,
+                                                                   uint32_t n_bins,
+                                                                   const float4* __restrict__ palette) {
+  __shared__ TileSumsPacked s_t;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t t = a.psegs[blockIdx.x];
+  const uint32_t base = t << SORT_TILE_SHIFT;
+  const uint32_t cap = a.tcap[t];
+  const uint32_t cur = a.tcur[t];
+  const uint32_t n = cur < cap ? cur : cap;  // <= SORT_SEG_PACKED: every field stays below 2^32
+  if (base >= n_bins || n == 0) return;     // workgroup-uniform
+  for (uint32_t i = tid; i < (uint32_t)TILE_BINS; i += TILE_THREADS) {
+    s_t.rg[i] = 0ull;
+    s_t.bc[i] = 0ull;
+  }
+  if (tid < 256u) {
+    const float4 c = palette[tid];
+    s_t.prg[tid] = (uint64_t)fix16(c.x) | ((uint64_t)fix16(c.y) << 32);
+    s_t.pb[tid] = (uint64_t)fix16(c.z) | (1ull << 32);
+  }
+  __syncthreads();
+  const uint32_t* src = a.tbuf + a.toff[t];  // 16-byte aligned (tile offsets are multiples of 4)
+  const uint4* src4 = reinterpret_cast<const uint4*>(src);
+  const uint32_t n4 = n / 4;
+  constexpr uint32_t G = 4;
+  for (uint32_t i0 = 0; i0 < n4; i0 += G * TILE_THREADS) {
+    uint4 q[G];
+#pragma unroll
+    for (uint32_t k = 0; k < G; ++k) {
+      const uint32_t i = i0 + k * TILE_THREADS + tid;
+      q[k] = i < n4 ? src4[i] : make_uint4(SORT_NONE, SORT_NONE, SORT_NONE, SORT_NONE);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < G; ++k) {
+      if (q[k].x != SORT_NONE) tile_add_packed(s_t, q[k].x);
+      if (q[k].y != SORT_NONE) tile_add_packed(s_t, q[k].y);
+      if (q[k].z != SORT_NONE) tile_add_packed(s_t, q[k].z);
+      if (q[k].w != SORT_NONE) tile_add_packed(s_t, q[k].w);
+    }
+  }
+  for (uint32_t i = n4 * 4 + tid; i < n; i += TILE_THREADS) tile_add_packed(s_t, src[i]);
+  __syncthreads();
+  constexpr float INV = 1.0f / 65536.0f;
+  for (uint32_t i = tid; i < (uint32_t)TILE_BINS; i += TILE_THREADS) {
+    const uint32_t bin = base + i;
+    const uint64_t bc = s_t.bc[i];
+    const uint32_t c = (uint32_t)(bc >> 32);
+    if (bin < n_bins && c != 0u) {
+      const uint64_t rg = s_t.rg[i];
+      float4 h = hist[bin];
+      h.x += (float)(uint32_t)rg * INV;
+      h.y += (float)(uint32_t)(rg >> 32) * INV;
+      h.z += (float)(uint32_t)bc * INV;
+      h.w += (float)c;
+      hist[bin] = h;
+    }
+  }
+}
+
 }  // namespace
 
 hipError_t launch_sort_reset(const SortArgs& a, hipStream_t s) {
@@ -217,6 +294,12 @@ hipError_t launch_sort_rebucket(const SortArgs& a, float* bins, const float4* pa
 }
 
 hipError_t launch_sort_tiles(const SortArgs& a, float* bins, uint32_t n_bins, const float4* palette, hipStream_t s) {
+  if (a.n_psegs) {
+    hipLaunchKernelGGL(k_sort_tiles_packed, dim3(a.n_psegs), dim3(TILE_THREADS), 0, s, a,
+                       reinterpret_cast<float4*>(bins), n_bins, palette);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   if (a.n_segs == 0) return hipSuccess;
   hipLaunchKernelGGL(k_sort_tiles, dim3(a.n_segs), dim3(TILE_THREADS), 0, s, a, reinterpret_cast<float4*>(bins),
                      n_bins, palette);

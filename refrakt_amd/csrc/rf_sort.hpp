@@ -21,6 +21,12 @@ constexpr int SORT_MAX_BUCKETS = 256;                                        // 
 constexpr int SORT_STAGE_SLOTS = 2048;                                       // samples staged in LDS per flush
 constexpr int SORT_CHUNK = 8192;                                             // records per rebucketing workgroup
 constexpr int SORT_SEG = 131072;                                             // records per tile-summing workgroup
+// Tiles whose capacity is at most SORT_SEG_PACKED records are summed by one
+// workgroup with two 64-bit LDS atomics per record: red | green << 32 and
+// blue | count << 32, colours in fixed point with 2^16 per unit (a field
+// holds at most 65532 * 65536 < 2^32, so no carry crosses into the next),
+// against three 64-bit colour atomics and a 32-bit count for larger tiles.
+constexpr int SORT_SEG_PACKED = 65532;
 constexpr uint32_t SORT_NONE = 0xffffffffu;
 
 struct SortArgs {
@@ -34,9 +40,10 @@ struct SortArgs {
   const uint32_t* tcap;    // per tile: capacity
   const uint32_t* chunks;  // rebucketing work: (bucket << 16) | chunk index, one per workgroup
   const uint32_t* segs;    // tile work: (tile << 16) | segment index, one per workgroup
+  const uint32_t* psegs;   // packed tile work (capacity <= SORT_SEG_PACKED): tile index
   uint32_t n_buckets;
   uint32_t n_tiles;        // n_buckets * SORT_TILES_PER_BUCKET
-  uint32_t n_chunks, n_segs;
+  uint32_t n_chunks, n_segs, n_psegs;
 };
 
 #ifndef __HIPCC_RTC__
