@@ -492,6 +492,35 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
   const uint32_t nb = (uint32_t)((n_bins + (1u << rfk::SORT_COARSE_SHIFT) - 1) >> rfk::SORT_COARSE_SHIFT);
   const uint32_t nt = nb * rfk::SORT_TILES_PER_BUCKET;
   const uint32_t nt_used = (uint32_t)((n_bins + (1u << rfk::SORT_TILE_SHIFT) - 1) >> rfk::SORT_TILE_SHIFT);
+  // Buffers that only grow, shared by the context's flames.
+  auto bind = [&](const uint32_t* dm, size_t fixed, uint32_t n_chunks, uint32_t n_segs, uint32_t n_psegs) {
+    a.cbuf = (uint32_t*)ctx->sort_cbuf.p;
+    a.tbuf = (uint32_t*)ctx->sort_tbuf.p;
+    a.bcur = (uint32_t*)ctx->sort_bcur.p;
+    a.tcur = (uint32_t*)ctx->sort_tcur.p;
+    a.boff = dm;
+    a.bcap = dm + nb;
+    a.toff = dm + 2 * (size_t)nb;
+    a.tcap = dm + 2 * (size_t)nb + nt;
+    a.chunks = dm + fixed;
+    a.segs = dm + fixed + n_chunks;
+    a.psegs = dm + fixed + n_chunks + n_segs;
+    a.n_buckets = nb;
+    a.n_tiles = nt;
+    a.n_chunks = n_chunks;
+    a.n_segs = n_segs;
+    a.n_psegs = n_psegs;
+  };
+  // Same draw size, same histogram and no overflow last time: the capacities
+  // and work lists on the device still fit.
+  if (!f->sort_meta.empty() && f->sort_meta_S == S && f->sort_meta_bins == n_bins && f->sort_bins == n_bins &&
+      f->last_sort_overflow == 0.0) {
+    meta = f->sort_meta;
+    bind((const uint32_t*)f->sort_meta_dev.p, f->sort_meta_fixed, f->sort_meta_chunks, f->sort_meta_segs,
+         f->sort_meta_psegs);
+    max_bcap = 0;
+    return RF_OK;
+  }
   if (f->sort_bins != n_bins || f->sort_bfrac.size() != nb) {
     f->sort_bins = n_bins;
     f->sort_bfrac.assign(nb, 3.0 / nb);
@@ -548,25 +577,16 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
   if (ctx->sort_bcur.bytes < (size_t)nb * 4) HIPCHK(ctx, ctx->sort_bcur.alloc((size_t)rfk::SORT_MAX_BUCKETS * 4));
   if (ctx->sort_tcur.bytes < (size_t)nt * 4)
     HIPCHK(ctx, ctx->sort_tcur.alloc((size_t)rfk::SORT_MAX_BUCKETS * rfk::SORT_TILES_PER_BUCKET * 4));
-  if (ctx->sort_meta.bytes < meta.size() * 4) HIPCHK(ctx, ctx->sort_meta.alloc(meta.size() * 4 * 2));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->sort_meta.p, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-  const uint32_t* dm = (const uint32_t*)ctx->sort_meta.p;
-  a.cbuf = (uint32_t*)ctx->sort_cbuf.p;
-  a.tbuf = (uint32_t*)ctx->sort_tbuf.p;
-  a.bcur = (uint32_t*)ctx->sort_bcur.p;
-  a.tcur = (uint32_t*)ctx->sort_tcur.p;
-  a.boff = dm;
-  a.bcap = dm + nb;
-  a.toff = dm + 2 * (size_t)nb;
-  a.tcap = dm + 2 * (size_t)nb + nt;
-  a.chunks = dm + meta_fixed;
-  a.segs = dm + meta_fixed + n_chunks;
-  a.psegs = dm + meta_fixed + n_chunks + n_segs;
-  a.n_psegs = n_psegs;
-  a.n_buckets = nb;
-  a.n_tiles = nt;
-  a.n_chunks = n_chunks;
-  a.n_segs = n_segs;
+  if (f->sort_meta_dev.bytes < meta.size() * 4) HIPCHK(ctx, f->sort_meta_dev.alloc(meta.size() * 4 * 2));
+  HIPCHK(ctx, hipMemcpyAsync(f->sort_meta_dev.p, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  bind((const uint32_t*)f->sort_meta_dev.p, meta_fixed, n_chunks, n_segs, n_psegs);
+  f->sort_meta = meta;
+  f->sort_meta_S = S;
+  f->sort_meta_bins = n_bins;
+  f->sort_meta_fixed = meta_fixed;
+  f->sort_meta_chunks = n_chunks;
+  f->sort_meta_segs = n_segs;
+  f->sort_meta_psegs = n_psegs;
   (void)max_bcap;
   return RF_OK;
 }

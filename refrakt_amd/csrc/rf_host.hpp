@@ -81,7 +81,7 @@ struct rf_ctx {
   int perm_waves = 0;
   rfh::DevBuf counter;         // u64 [drawn, samples added in the LDS bin cache]
   // RF_HIST_SORT buffers: bucket and tile records, cursors, offsets, capacities
-  rfh::DevBuf sort_cbuf, sort_tbuf, sort_bcur, sort_tcur, sort_meta;
+  rfh::DevBuf sort_cbuf, sort_tbuf, sort_bcur, sort_tcur;
   rfh::DevBuf probe;           // atomic probe scratch
   rf_hist* render_hist = nullptr;
   rfh::DevBuf render_img;
@@ -114,6 +114,13 @@ struct rf_flame {
   uint32_t sort_bins = 0;
   std::vector<double> sort_bfrac, sort_tfrac;
   double last_sort_overflow = -1.0;
+  // The last sort draw's capacities and work lists (host copy and device
+  // copy), reused while draws keep the same size and nothing overflows.
+  std::vector<uint32_t> sort_meta;
+  rfh::DevBuf sort_meta_dev;
+  uint64_t sort_meta_S = 0;
+  uint32_t sort_meta_bins = 0, sort_meta_chunks = 0, sort_meta_segs = 0, sort_meta_psegs = 0;
+  size_t sort_meta_fixed = 0;
   // Flame program: the host copy of the Block (its structure selects the
   // program), whether the last fast-engine launch ran one.
   std::vector<rfd::Block> host_block;
