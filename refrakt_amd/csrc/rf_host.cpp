@@ -633,7 +633,8 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
   const uint64_t P = ctx->sp.total_particles;
   float ss[6];
   screen_affine(f->desc, h->w, h->h, ss);
-  HIPCHK(ctx, hipMemsetAsync(ctx->counter.p, 0, 16, ctx->stream));
+  constexpr size_t counter_bytes = 8ull * rfk::COUNTER_SLOTS * rfk::COUNTER_STRIDE;
+  HIPCHK(ctx, hipMemsetAsync(ctx->counter.p, 0, counter_bytes, ctx->stream));
   const int hm = ctx->sp.hist_mode == RF_HIST_AUTO ? f->auto_mode : ctx->sp.hist_mode;
   bool sort_used = false;
   rfk::SortArgs sort_args{};
@@ -698,8 +699,13 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   uint64_t cnt[2] = {0, 0};
-  HIPCHK(ctx, hipMemcpyAsync(cnt, ctx->counter.p, 16, hipMemcpyDeviceToHost, ctx->stream));
+  uint64_t slots[rfk::COUNTER_SLOTS * rfk::COUNTER_STRIDE];
+  HIPCHK(ctx, hipMemcpyAsync(slots, ctx->counter.p, counter_bytes, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < rfk::COUNTER_SLOTS; ++k) {
+    cnt[0] += slots[k * rfk::COUNTER_STRIDE];
+    cnt[1] += slots[k * rfk::COUNTER_STRIDE + 1];
+  }
   if (sort_used) {
     const int rc2 = sort_finish(ctx, f, sort_args, sort_S, cnt[0], sort_meta);
     if (rc2) return rc2;
@@ -790,7 +796,7 @@ int rf_create(const rf_sim_params* p, rf_ctx** out) {
   // kernels read or write them, and the caller's later work sees our results.
   e = hipStreamCreate(&ctx->stream);
   if (e != hipSuccess) { g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e); delete ctx; return RF_ERR_HIP; }
-  e = ctx->counter.alloc(16);
+  e = ctx->counter.alloc(8ull * rfk::COUNTER_SLOTS * rfk::COUNTER_STRIDE);
   if (e != hipSuccess) { g_create_err = std::string("hipMalloc: ") + hipGetErrorString(e); rf_destroy(ctx); return RF_ERR_HIP; }
   ctx->ppt = p->total_particles / p->temporal_samples;
   ctx->shuffle_count = ctx->sp.shuffle_count;

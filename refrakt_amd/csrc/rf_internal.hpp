@@ -64,6 +64,13 @@ struct DEArgs {
 };
 
 constexpr int NPERM = 16;         // permutation rows of the fast engine (held in LDS)
+// Draw counters: COUNTER_SLOTS slots of COUNTER_STRIDE u64 (one 128-byte
+// line each; slot s holds [binned samples, LDS-table re-hits]).  Memory-side
+// atomics on one address serialise (about 80 M/s): one counter for every
+// wave of a 2^24-particle draw cost 1.6 ms, spread over 64 lines they cost
+// nothing.  The host sums the slots.
+constexpr int COUNTER_SLOTS = 64;
+constexpr int COUNTER_STRIDE = 16;
 
 #ifndef __HIPCC_RTC__
 hipError_t launch_init_rs(uint4* rs, uint64_t n, hipStream_t s);

@@ -226,8 +226,7 @@ __device__ __forceinline__ void k_fast_body(const FastArgs& g, const Block* __re
     if constexpr (HM == 4) {
       __syncthreads();
       flush_lds(s_bins, g.bins);
-      const unsigned long long wh = (unsigned long long)wave_sum_u32(hits);
-      if ((threadIdx.x & (WAVE - 1)) == 0 && wh) atomicAdd(g.counter + 1, wh);
+      count_slot(g.counter, 1, (unsigned long long)wave_sum_u32(hits));
     }
   }
 }

@@ -344,9 +344,14 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Adds a wave-uniform count to field `field` of this wave's counter slot.
+__device__ __forceinline__ void count_slot(unsigned long long* counter, int field, unsigned long long n) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const uint32_t slot = (blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) % (uint32_t)COUNTER_SLOTS;
+  if (lane == 0 && n) atomicAdd(counter + slot * COUNTER_STRIDE + field, n);
+}
 __device__ __forceinline__ void count_drawn(unsigned long long* counter, unsigned long long n) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  if (lane == 0 && n) atomicAdd(counter, n);
+  count_slot(counter, 0, n);
 }
 
 }  // namespace detail
