@@ -28,25 +28,30 @@ using namespace rfd;
 // HM 1/2/3/5 launch one workgroup per batch; HM 4 (LDS bin cache) launches
 // one workgroup per resident slot so that each workgroup's LDS table spans
 // many batches before it is flushed.
-// HIST_SORT staging: the samples of up to SORT_STAGE_SLOTS / N iterations wait in
-// LDS (one fixed slot per lane and iteration, no append atomics).  A flush
-// ranks them by bucket (counting sort), reserves one run per non-empty bucket
-// with one global atomic, places the records in bucket order in LDS and
-// writes them out so that neighbouring lanes store neighbouring addresses.
+// HIST_SORT staging: the samples of SI = sort_stage_iters<N>() iterations
+// wait in LDS (one fixed slot per lane and iteration, no append atomics).  A
+// flush ranks them by bucket (LDS counting sort; each lane loads its staged
+// bins first, then issues its rank atomics back to back and keeps the ranks
+// in registers), wave 0 scans the counts, reserves one run per non-empty
+// bucket with one global atomic and clears the counts, the records are placed
+// in bucket order in LDS and written out so that neighbouring lanes store
+// neighbouring addresses.  Three barriers per flush.
 template <int N>
 constexpr int sort_stage_iters() { return SORT_STAGE_SLOTS / N; }
+template <int N>
+constexpr int sort_stage_slots() { return sort_stage_iters<N>() * N; }
 
 template <int N, bool ON>
 struct SortStage {
-  static constexpr int S = SORT_STAGE_SLOTS;
-  uint32_t bin[S];
+  static constexpr int S = sort_stage_slots<N>();
+  uint32_t bin[S];       // staged bin index (SORT_NONE: no sample), slot = iteration * N + lane
   uint32_t rec[S];       // bucket-ordered records
-  uint16_t rank[S];
-  uint8_t pid[S];
+  uint8_t pid[S];        // staged palette index
   uint8_t bkt[S];        // bucket of rec[j]
-  uint32_t cnt[SORT_MAX_BUCKETS];
-  uint32_t off[SORT_MAX_BUCKETS];
-  uint32_t gb[SORT_MAX_BUCKETS];
+  uint32_t cnt[SORT_MAX_BUCKETS];    // records per bucket in this flush (zero between flushes)
+  uint32_t off[SORT_MAX_BUCKETS];    // exclusive scan of cnt
+  uint32_t base[SORT_MAX_BUCKETS];   // record j of bucket b goes to cbuf[base[b] + j] ...
+  uint32_t lim[SORT_MAX_BUCKETS];    // ... if j < lim[b], else over the bucket's capacity
   uint32_t boff[SORT_MAX_BUCKETS];   // bucket offsets and capacities, staged once per workgroup
   uint32_t bcap[SORT_MAX_BUCKETS];
   uint32_t total;
@@ -55,33 +60,62 @@ template <int N>
 struct SortStage<N, false> {};
 
 template <int N>
-__device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_slots, const SortArgs& a,
+__device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_staged, const SortArgs& a,
                                            float* __restrict__ bins, const float4* __restrict__ pal) {
-  static_assert(SORT_MAX_BUCKETS <= 4 * WAVE, "scan width");
+  constexpr int SI = sort_stage_iters<N>();
+  static_assert(SORT_MAX_BUCKETS % WAVE == 0, "scan segments");
   const uint32_t tid = threadIdx.x;
-  for (uint32_t b = tid; b < a.n_buckets; b += N) st.cnt[b] = 0u;
+  // 1. rank this lane's staged samples within their buckets
+  uint32_t sb[SI], rank[SI];
+#pragma unroll
+  for (int k = 0; k < SI; ++k) sb[k] = (uint32_t)k < n_staged ? st.bin[k * N + tid] : SORT_NONE;
+#pragma unroll
+  for (int k = 0; k < SI; ++k)
+    rank[k] = sb[k] != SORT_NONE ? atomicAdd(&st.cnt[sb[k] >> SORT_COARSE_SHIFT], 1u) : 0u;
   lds_barrier();
-  for (uint32_t i = tid; i < n_slots; i += N) {
-    const uint32_t bin = st.bin[i];
-    if (bin != SORT_NONE) st.rank[i] = (uint16_t)atomicAdd(&st.cnt[bin >> SORT_COARSE_SHIFT], 1u);
+  // 2. wave 0: exclusive scan of the counts (64-bucket segments, lane = bucket
+  //    within the segment), one reservation atomic per non-empty bucket,
+  //    destinations, and the counts cleared for the next flush
+  if (tid < (uint32_t)WAVE) {
+    uint32_t carry = 0;
+    for (uint32_t s0 = 0; s0 < a.n_buckets; s0 += WAVE) {  // uniform
+      const uint32_t b = s0 + tid;
+      const bool in = b < a.n_buckets;
+      const uint32_t v = in ? st.cnt[b] : 0u;
+      const uint32_t gb = v ? atomicAdd(a.bcur + b, v) : 0u;
+      uint32_t inc = v;
+#pragma unroll
+      for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)inc, d);
+        if ((int)tid >= d) inc += t;
+      }
+      const uint32_t run = carry + inc - v;
+      if (in) {
+        const uint32_t cap = st.bcap[b];
+        st.off[b] = run;
+        st.base[b] = st.boff[b] + gb - run;
+        st.lim[b] = run + (cap > gb ? cap - gb : 0u);
+        st.cnt[b] = 0u;
+      }
+      carry += (uint32_t)__shfl((int)inc, WAVE - 1);
+    }
+    if (tid == 0) st.total = carry;
   }
   lds_barrier();
-  wave0_exclusive_scan<4>(st.cnt, st.off, a.n_buckets, &st.total);
-  for (uint32_t b = tid; b < a.n_buckets; b += N) {
-    const uint32_t c = st.cnt[b];
-    st.gb[b] = c ? atomicAdd(a.bcur + b, c) : 0u;
-  }
-  lds_barrier();
-  for (uint32_t i = tid; i < n_slots; i += N) {
-    const uint32_t bin = st.bin[i];
-    if (bin != SORT_NONE) {
-      const uint32_t b = bin >> SORT_COARSE_SHIFT;
-      const uint32_t j = st.off[b] + st.rank[i];
-      st.rec[j] = ((bin & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | st.pid[i];
+  // 3. place the records in bucket order
+#pragma unroll
+  for (int k = 0; k < SI; ++k) {
+    if (sb[k] != SORT_NONE) {
+      const uint32_t b = sb[k] >> SORT_COARSE_SHIFT;
+      const uint32_t j = st.off[b] + rank[k];
+      st.rec[j] = ((sb[k] & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | st.pid[k * N + tid];
       st.bkt[j] = (uint8_t)b;
     }
   }
   lds_barrier();
+  // 4. coalesced write-out; records beyond a bucket's capacity go straight to
+  //    the histogram.  Later writes to what this step reads (rec, bkt, base,
+  //    lim, total) come after the next flush's first barrier.
   const uint32_t total = st.total;
   const uint32_t rounds = (total + N - 1) / N;  // uniform
   for (uint32_t r = 0; r < rounds; ++r) {
@@ -91,10 +125,9 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_sl
     if (j < total) {
       const uint32_t b = st.bkt[j];
       const uint32_t rec = st.rec[j];
-      const uint32_t dst = st.gb[b] + (j - st.off[b]);
-      if (dst < st.bcap[b]) {
-        a.cbuf[st.boff[b] + dst] = rec;
-      } else {  // over the bucket's capacity: straight to the histogram
+      if (j < st.lim[b]) {
+        a.cbuf[st.base[b] + j] = rec;
+      } else {
         ovf = true;
         bin = (b << SORT_COARSE_SHIFT) | (rec >> 8);
         pidx = rec & 255u;
@@ -102,7 +135,6 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_sl
     }
     bin_quad(bins, bin, ovf, pidx, 1.0f, pal);
   }
-  lds_barrier();
 }
 
 template <int WAVES, int HM, int LOGK, class Prog>
@@ -130,6 +162,7 @@ __device__ __forceinline__ void k_fast_body(const FastArgs& g, const Block* __re
       for (uint32_t b = tid; b < g.sort.n_buckets; b += N) {
         s_sort.boff[b] = g.sort.boff[b];
         s_sort.bcap[b] = g.sort.bcap[b];
+        s_sort.cnt[b] = 0u;
       }
     }
   }
@@ -213,7 +246,7 @@ __device__ __forceinline__ void k_fast_body(const FastArgs& g, const Block* __re
         // the exchange barrier above also orders the staging writes before the flush
         constexpr uint32_t SI = sort_stage_iters<N>();
         if (g.do_draw && ((it % SI) == SI - 1 || it + 1 == g.n_iter))
-          sort_flush(s_sort, ((it % SI) + 1) * (uint32_t)N, g.sort, g.bins, s_pal);
+          sort_flush(s_sort, (it % SI) + 1, g.sort, g.bins, s_pal);
       }
     }
     g.pos[pid] = make_float4(x, y, c, 0.0f);
