@@ -18,7 +18,7 @@ constexpr int SORT_COARSE_SHIFT = 19;                                        // 
 constexpr int SORT_TILE_SHIFT = 11;                                          // bins per tile: 2^11
 constexpr int SORT_TILES_PER_BUCKET = 1 << (SORT_COARSE_SHIFT - SORT_TILE_SHIFT);  // 256
 constexpr int SORT_MAX_BUCKETS = 256;                                        // 2^27 bins
-constexpr int SORT_STAGE_SLOTS = 2048;                                       // samples staged in LDS per flush
+constexpr int SORT_STAGE_SLOTS = 4096;                                       // samples staged in LDS per flush
 constexpr int SORT_CHUNK = 8192;                                             // records per rebucketing workgroup
 constexpr int SORT_SEG = 131072;                                             // records per tile-summing workgroup
 // Tiles whose capacity is at most SORT_SEG_PACKED records are summed by one
