@@ -77,24 +77,32 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_st
   //    within the segment), one reservation atomic per non-empty bucket,
   //    destinations, and the counts cleared for the next flush
   if (tid < (uint32_t)WAVE) {
+    constexpr int SEG = SORT_MAX_BUCKETS / WAVE;
+    uint32_t v[SEG], gb[SEG];
+    // all reservation atomics first (one round trip), then the scans
+#pragma unroll
+    for (int k = 0; k < SEG; ++k) {
+      const uint32_t b = k * WAVE + tid;
+      v[k] = b < a.n_buckets ? st.cnt[b] : 0u;
+      gb[k] = v[k] ? atomicAdd(a.bcur + b, v[k]) : 0u;
+    }
     uint32_t carry = 0;
-    for (uint32_t s0 = 0; s0 < a.n_buckets; s0 += WAVE) {  // uniform
-      const uint32_t b = s0 + tid;
-      const bool in = b < a.n_buckets;
-      const uint32_t v = in ? st.cnt[b] : 0u;
-      const uint32_t gb = v ? atomicAdd(a.bcur + b, v) : 0u;
-      uint32_t inc = v;
+#pragma unroll
+    for (int k = 0; k < SEG; ++k) {
+      if ((uint32_t)(k * WAVE) >= a.n_buckets) break;  // uniform
+      const uint32_t b = k * WAVE + tid;
+      uint32_t inc = v[k];
 #pragma unroll
       for (int d = 1; d < WAVE; d <<= 1) {
         const uint32_t t = (uint32_t)__shfl_up((int)inc, d);
         if ((int)tid >= d) inc += t;
       }
-      const uint32_t run = carry + inc - v;
-      if (in) {
+      const uint32_t run = carry + inc - v[k];
+      if (b < a.n_buckets) {
         const uint32_t cap = st.bcap[b];
         st.off[b] = run;
-        st.base[b] = st.boff[b] + gb - run;
-        st.lim[b] = run + (cap > gb ? cap - gb : 0u);
+        st.base[b] = st.boff[b] + gb[k] - run;
+        st.lim[b] = run + (cap > gb[k] ? cap - gb[k] : 0u);
         st.cnt[b] = 0u;
       }
       carry += (uint32_t)__shfl((int)inc, WAVE - 1);
