@@ -493,7 +493,7 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
   const uint32_t nt = nb * rfk::SORT_TILES_PER_BUCKET;
   const uint32_t nt_used = (uint32_t)((n_bins + (1u << rfk::SORT_TILE_SHIFT) - 1) >> rfk::SORT_TILE_SHIFT);
   // Buffers that only grow, shared by the context's flames.
-  auto bind = [&](const uint32_t* dm, size_t fixed, uint32_t n_chunks, uint32_t n_segs, uint32_t n_psegs) {
+  auto bind = [&](const uint32_t* dm, size_t fixed, uint32_t n_chunks, uint32_t n_segs) {
     a.cbuf = (uint32_t*)ctx->sort_cbuf.p;
     a.tbuf = (uint32_t*)ctx->sort_tbuf.p;
     a.bcur = (uint32_t*)ctx->sort_bcur.p;
@@ -504,20 +504,17 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
     a.tcap = dm + 2 * (size_t)nb + nt;
     a.chunks = dm + fixed;
     a.segs = dm + fixed + n_chunks;
-    a.psegs = dm + fixed + n_chunks + n_segs;
     a.n_buckets = nb;
     a.n_tiles = nt;
     a.n_chunks = n_chunks;
     a.n_segs = n_segs;
-    a.n_psegs = n_psegs;
   };
   // Same draw size, same histogram and no overflow last time: the capacities
   // and work lists on the device still fit.
   if (!f->sort_meta.empty() && f->sort_meta_S == S && f->sort_meta_bins == n_bins && f->sort_bins == n_bins &&
       f->last_sort_overflow == 0.0) {
     meta = f->sort_meta;
-    bind((const uint32_t*)f->sort_meta_dev.p, f->sort_meta_fixed, f->sort_meta_chunks, f->sort_meta_segs,
-         f->sort_meta_psegs);
+    bind((const uint32_t*)f->sort_meta_dev.p, f->sort_meta_fixed, f->sort_meta_chunks, f->sort_meta_segs);
     max_bcap = 0;
     return RF_OK;
   }
@@ -560,17 +557,21 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
       jobs.push_back((b << 16) | k);
     }
   const uint32_t n_chunks = (uint32_t)jobs.size();
-  std::vector<uint32_t> packed;
+  // tile segments, the largest first so that the long workgroups start early
+  std::vector<std::pair<uint32_t, uint32_t>> segs;  // (records the segment can hold, job)
   for (uint32_t t = 0; t < nt_used; ++t) {
-    if (tcap[t] <= (uint32_t)rfk::SORT_SEG_PACKED) { packed.push_back(t); continue; }
-    for (uint32_t k = 0; k * (uint64_t)rfk::SORT_SEG < tcap[t]; ++k) {
+    if (tcap[t] <= (uint32_t)rfk::SORT_SEG16) {
+      segs.emplace_back(tcap[t], t << 16);
+      continue;
+    }
+    for (uint32_t k = 0; k * (uint64_t)rfk::SORT_SEG15 < tcap[t]; ++k) {
       if (k > 0xffffu) return set_err(ctx, RF_ERR_LIMIT, "sort histogram: tile too large");
-      jobs.push_back((t << 16) | k);
+      segs.emplace_back(std::min<uint32_t>(tcap[t] - k * (uint32_t)rfk::SORT_SEG15, rfk::SORT_SEG15), (t << 16) | k);
     }
   }
-  const uint32_t n_segs = (uint32_t)jobs.size() - n_chunks;
-  const uint32_t n_psegs = (uint32_t)packed.size();
-  jobs.insert(jobs.end(), packed.begin(), packed.end());
+  std::stable_sort(segs.begin(), segs.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+  for (const auto& sg : segs) jobs.push_back(sg.second);
+  const uint32_t n_segs = (uint32_t)segs.size();
   meta.insert(meta.end(), jobs.begin(), jobs.end());
   if (ctx->sort_cbuf.bytes < cb * 4) HIPCHK(ctx, ctx->sort_cbuf.alloc(cb * 4));
   if (ctx->sort_tbuf.bytes < ct * 4) HIPCHK(ctx, ctx->sort_tbuf.alloc(ct * 4));
@@ -579,14 +580,13 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
     HIPCHK(ctx, ctx->sort_tcur.alloc((size_t)rfk::SORT_MAX_BUCKETS * rfk::SORT_TILES_PER_BUCKET * 4));
   if (f->sort_meta_dev.bytes < meta.size() * 4) HIPCHK(ctx, f->sort_meta_dev.alloc(meta.size() * 4 * 2));
   HIPCHK(ctx, hipMemcpyAsync(f->sort_meta_dev.p, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-  bind((const uint32_t*)f->sort_meta_dev.p, meta_fixed, n_chunks, n_segs, n_psegs);
+  bind((const uint32_t*)f->sort_meta_dev.p, meta_fixed, n_chunks, n_segs);
   f->sort_meta = meta;
   f->sort_meta_S = S;
   f->sort_meta_bins = n_bins;
   f->sort_meta_fixed = meta_fixed;
   f->sort_meta_chunks = n_chunks;
   f->sort_meta_segs = n_segs;
-  f->sort_meta_psegs = n_psegs;
   (void)max_bcap;
   return RF_OK;
 }

@@ -119,7 +119,7 @@ struct rf_flame {
   std::vector<uint32_t> sort_meta;
   rfh::DevBuf sort_meta_dev;
   uint64_t sort_meta_S = 0;
-  uint32_t sort_meta_bins = 0, sort_meta_chunks = 0, sort_meta_segs = 0, sort_meta_psegs = 0;
+  uint32_t sort_meta_bins = 0, sort_meta_chunks = 0, sort_meta_segs = 0;
   size_t sort_meta_fixed = 0;
   // Flame program: the host copy of the Block (its structure selects the
   // program), whether the last fast-engine launch ran one.

@@ -20,13 +20,15 @@ constexpr int SORT_TILES_PER_BUCKET = 1 << (SORT_COARSE_SHIFT - SORT_TILE_SHIFT)
 constexpr int SORT_MAX_BUCKETS = 256;                                        // 2^27 bins
 constexpr int SORT_STAGE_SLOTS = 4096;                                       // samples staged in LDS per flush
 constexpr int SORT_CHUNK = 8192;                                             // records per rebucketing workgroup
-constexpr int SORT_SEG = 131072;                                             // records per tile-summing workgroup
-// Tiles whose capacity is at most SORT_SEG_PACKED records are summed by one
-// workgroup with two 64-bit LDS atomics per record: red | green << 32 and
-// blue | count << 32, colours in fixed point with 2^16 per unit (a field
-// holds at most 65532 * 65536 < 2^32, so no carry crosses into the next),
-// against three 64-bit colour atomics and a 32-bit count for larger tiles.
-constexpr int SORT_SEG_PACKED = 65532;
+// Tile sums add red | green << 32 and blue | count << 32 per record with
+// 64-bit LDS atomics, colours in fixed point.  A tile of at most SORT_SEG16
+// records is summed by one workgroup at 2^16 per unit (a field then holds at
+// most 65532 * 65536 < 2^32, so no carry crosses into the next); a larger
+// tile is split into segments of at most SORT_SEG15 records, each summed by
+// its own workgroup at 2^15 per unit (131068 * 32768 < 2^32).  Each sample's
+// colour is rounded by at most 2^-17 or 2^-16.
+constexpr int SORT_SEG16 = 65532;
+constexpr int SORT_SEG15 = 131068;
 constexpr uint32_t SORT_NONE = 0xffffffffu;
 
 struct SortArgs {
@@ -40,10 +42,9 @@ struct SortArgs {
   const uint32_t* tcap;    // per tile: capacity
   const uint32_t* chunks;  // rebucketing work: (bucket << 16) | chunk index, one per workgroup
   const uint32_t* segs;    // tile work: (tile << 16) | segment index, one per workgroup
-  const uint32_t* psegs;   // packed tile work (capacity <= SORT_SEG_PACKED): tile index
   uint32_t n_buckets;
   uint32_t n_tiles;        // n_buckets * SORT_TILES_PER_BUCKET
-  uint32_t n_chunks, n_segs, n_psegs;
+  uint32_t n_chunks, n_segs;
 };
 
 #ifndef __HIPCC_RTC__
