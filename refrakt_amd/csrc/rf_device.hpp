@@ -410,10 +410,13 @@ template <uint32_t... C>
 struct TermList {};
 
 // Applies xform `xi` whose structure is fixed at compile time: the precalc
-// mask PCM, whether it has a post-affine, and its pre_xform and regular terms
-// in Block order.  Coefficients, weights and parameters are still read from
-// the Block, so a flame's values can change without a new program.
-template <uint32_t PCM, bool POST, class Rng, class BP, uint32_t... PRE, uint32_t... REG>
+// mask PCM, whether it has a post-affine, the index of its first term in the
+// Block (TB; TERM_BEGIN_LOAD: read it from the Block) and its pre_xform and
+// regular terms in Block order.  Coefficients, weights and parameters are
+// still read from the Block, so a flame's values can change without a new
+// program; with TB fixed, none of those loads waits for another.
+constexpr uint32_t TERM_BEGIN_LOAD = 0xffffffffu;
+template <uint32_t PCM, bool POST, uint32_t TB, class Rng, class BP, uint32_t... PRE, uint32_t... REG>
 __device__ __forceinline__ XOut apply_fixed(TermList<PRE...>, TermList<REG...>, BP blk, int xi, float x0,
                                             float y0, float c0, bool first, Rng& rng) {
   const auto* __restrict__ xf = &blk->xf[xi];
@@ -422,7 +425,7 @@ __device__ __forceinline__ XOut apply_fixed(TermList<PRE...>, TermList<REG...>, 
   for (int k = 0; k < 6; ++k) A[k] = xf->aff[k];
   float x = fmaf(A[0], x0, fmaf(A[2], y0, A[4]));
   float y = fmaf(A[1], x0, fmaf(A[3], y0, A[5]));
-  const auto* __restrict__ T = blk->terms + xf->term_begin;
+  const auto* __restrict__ T = blk->terms + (TB == TERM_BEGIN_LOAD ? xf->term_begin : TB);
   Pre pc;
   pc.r = pc.rsq = pc.a = pc.phi = pc.sina = pc.cosa = pc.sinr = pc.cosr = 0.0f;
   uint32_t k = 0;
@@ -455,6 +458,23 @@ __device__ __forceinline__ int select_xform_raw(BP blk, uint32_t x_any) {
   int i = 0;
   while (i < n - 1 && !(((never >> i) & 1u) == 0u && x <= blk->cdf_lim[i])) ++i;
   return i;
+}
+
+// select_xform_raw for flames of at most NMAX xforms: the first NMAX - 1
+// limits are compared independently (their scalar loads issue together, no
+// load-compare-branch chain) into a mask of the xforms that would accept x,
+// and the lowest accepting one is taken; the same index as the loop.
+template <int NMAX, class BP>
+__device__ __forceinline__ int select_xform_upto(BP blk, uint32_t x_any) {
+  static_assert(NMAX >= 1 && NMAX <= MAX_XF, "xform count");
+  const uint32_t x = __builtin_amdgcn_readfirstlane(x_any);
+  const uint32_t n = blk->n_xforms;  // 1 <= n <= NMAX
+  uint32_t ok = 0u;
+#pragma unroll
+  for (int i = 0; i < NMAX - 1; ++i) ok |= (x <= blk->cdf_lim[i] ? 1u : 0u) << i;
+  const uint32_t last = 1u << (n - 1u);  // the last xform is unconditional
+  ok = (ok & ~blk->never_mask & (last - 1u)) | last;
+  return (int)__builtin_ctz(ok);
 }
 
 }  // namespace rfd

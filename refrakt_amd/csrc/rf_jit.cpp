@@ -36,7 +36,11 @@ std::string xform_key(const rfd::Block& b, int xi) {
 
 void xform_args(std::ostringstream& o, const rfd::Block& b, int xi) {
   const rfd::Xform& X = b.xf[xi];
-  o << "apply_fixed<" << X.precalc << "u, " << ((X.flags & rfd::XF_HAS_POST) ? "true" : "false") << ">(TermList<";
+  // A regular xform's terms start after those of the xforms before it, the
+  // same in every flame the program runs; the final xform's start depends on
+  // the flame's xform count and is read from the Block.
+  o << "apply_fixed<" << X.precalc << "u, " << ((X.flags & rfd::XF_HAS_POST) ? "true" : "false") << ", "
+    << (xi == rfd::FINAL ? std::string("TERM_BEGIN_LOAD") : std::to_string(X.term_begin) + "u") << ">(TermList<";
   for (uint32_t k = 0; k < X.n_pre; ++k) {
     const rfd::Term& T = b.terms[X.term_begin + k];
     o << (k ? ", " : "") << (T.var | (T.flags << 16)) << "u";
@@ -78,14 +82,16 @@ std::string program_source(const rfd::Block& b, const std::vector<std::pair<int,
     << "typedef unsigned long uint64_t;\ntypedef int int32_t;\ntypedef long int64_t;\n"
     << "#include \"rf_k_fast_body.hpp\"\n#include \"rf_k_batch_body.hpp\"\n"
     << "namespace rfprog {\nusing namespace rfd;\n"
-    << "struct Prog {\n"
-    << "  template <class BP>\n"
-    << "  static __device__ __forceinline__ int select(BP blk, uint32_t raw) { return select_xform_raw(blk, raw); }\n"
+    << "struct Prog {\n";
+  const int n = (int)b.n_xforms;
+  // flames run by this program have at most n xforms (see structure_covers)
+  o << "  template <class BP>\n"
+    << "  static __device__ __forceinline__ int select(BP blk, uint32_t raw) { return select_xform_upto<" << n
+    << ">(blk, raw); }\n"
     << "  template <class Rng, class BP>\n"
     << "  static __device__ __forceinline__ XOut apply(BP blk, int xi, float x, float y, float c, bool first,\n"
     << "                                              Rng& rng) {\n"
     << "    switch (xi) {\n";
-  const int n = (int)b.n_xforms;
   for (int i = 0; i < n; ++i) {
     o << "      " << (i + 1 < n ? "case " + std::to_string(i) + ":" : std::string("default:")) << " return ";
     xform_args(o, b, i);
