@@ -34,13 +34,14 @@ std::string xform_key(const rfd::Block& b, int xi) {
   return o.str();
 }
 
-void xform_args(std::ostringstream& o, const rfd::Block& b, int xi) {
+// An xform's terms start after those of the xforms before it: the same in
+// every flame with this structure, so the offset is a constant, except for
+// the final xform of the batched renderer (load_tb), whose programs also run
+// flames with fewer xforms (structure_covers).
+void xform_args(std::ostringstream& o, const rfd::Block& b, int xi, bool load_tb) {
   const rfd::Xform& X = b.xf[xi];
-  // A regular xform's terms start after those of the xforms before it, the
-  // same in every flame the program runs; the final xform's start depends on
-  // the flame's xform count and is read from the Block.
   o << "apply_fixed<" << X.precalc << "u, " << ((X.flags & rfd::XF_HAS_POST) ? "true" : "false") << ", "
-    << (xi == rfd::FINAL ? std::string("TERM_BEGIN_LOAD") : std::to_string(X.term_begin) + "u") << ">(TermList<";
+    << (load_tb ? std::string("TERM_BEGIN_LOAD") : std::to_string(X.term_begin) + "u") << ">(TermList<";
   for (uint32_t k = 0; k < X.n_pre; ++k) {
     const rfd::Term& T = b.terms[X.term_begin + k];
     o << (k ? ", " : "") << (T.var | (T.flags << 16)) << "u";
@@ -94,23 +95,31 @@ std::string program_source(const rfd::Block& b, const std::vector<std::pair<int,
     << "    switch (xi) {\n";
   for (int i = 0; i < n; ++i) {
     o << "      " << (i + 1 < n ? "case " + std::to_string(i) + ":" : std::string("default:")) << " return ";
-    xform_args(o, b, i);
+    xform_args(o, b, i, false);
     o << ", x, y, c, first, rng);\n";
   }
   o << "    }\n  }\n"
     << "  template <class BP>\n"
     << "  static __device__ __forceinline__ bool has_final(BP) { return " << (b.has_final ? "true" : "false")
-    << "; }\n"
-    << "  template <class Rng, class BP>\n"
-    << "  static __device__ __forceinline__ XOut apply_final(BP blk, float x, float y, float c, Rng& rng) {\n";
-  if (b.has_final) {
-    o << "    return ";
-    xform_args(o, b, rfd::FINAL);
-    o << ", x, y, c, false, rng);\n";
-  } else {
-    o << "    XOut r; r.x = x; r.y = y; r.c = c; r.w = 1.0f; return r;\n";
-  }
-  o << "  }\n};\n}  // namespace rfprog\n";
+    << "; }\n";
+  auto final_fn = [&](bool load_tb) {
+    o << "  template <class Rng, class BP>\n"
+      << "  static __device__ __forceinline__ XOut apply_final(BP blk, float x, float y, float c, Rng& rng) {\n";
+    if (b.has_final) {
+      o << "    return ";
+      xform_args(o, b, rfd::FINAL, load_tb);
+      o << ", x, y, c, false, rng);\n";
+    } else {
+      o << "    XOut r; r.x = x; r.y = y; r.c = c; r.w = 1.0f; return r;\n";
+    }
+    o << "  }\n";
+  };
+  final_fn(false);
+  o << "};\n"
+    << "// the batched renderer's view: the final xform's terms found through the Block\n"
+    << "struct BProg : Prog {\n";
+  final_fn(true);
+  o << "};\n}  // namespace rfprog\n";
   for (const auto& k : kernels) {
     const int W = k.first, mode = k.second;
     o << "extern \"C\" __global__ __launch_bounds__(" << 64 * W << ") void " << entry_name(W, mode);
@@ -118,7 +127,7 @@ std::string program_source(const rfd::Block& b, const std::vector<std::pair<int,
       o << "(rfk::BatchArgs g, const rfd::Block* __restrict__ blocks,\n"
         << "    const rfk::BatchFrame* __restrict__ frames) {\n"
         << "  rfk::detail::k_batch_body<" << W << ", " << mode - kBatchKernel << ", " << lds_logk(W, mode)
-        << ", rfprog::Prog>(g, blocks, frames);\n}\n";
+        << ", rfprog::BProg>(g, blocks, frames);\n}\n";
     } else {
       o << "(rfk::FastArgs g, const rfd::Block* __restrict__ blocks) {\n"
         << "  rfk::detail::k_fast_body<" << W << ", " << mode << ", " << lds_logk(W, mode)

@@ -63,8 +63,10 @@ def test_program_source_fixes_the_structure(rf):
     assert 'extern "C" __global__ __launch_bounds__(512) void rf_prog_w8_h5' in src
     assert "k_fast_body<8, 5, 0, rfprog::Prog>" in src
     # one switch case per xform (the last as default), the final xform inline
-    assert src.count("return apply_fixed<") == len(fl.xforms) + 1
+    # (once with its term offset fixed, once for the batched renderer)
+    assert src.count("return apply_fixed<") == len(fl.xforms) + 2
     assert "has_final(BP) { return true; }" in src
+    assert "TERM_BEGIN_LOAD" not in src.split("struct BProg")[0]
     lin = rf.variation_id("linear")
     assert f"TermList<{lin}u>(), blk, FINAL" in src
     with pytest.raises(rf.RefraktError):
@@ -142,6 +144,8 @@ def test_batch_program_compiles(rf):
     frames = configs.cfg4()
     big = max((f.flame for f in frames), key=lambda fl: len(fl.xforms))
     src = rf.program_source(big, 8, 16 + 4)
-    assert "rf_batch_w8_h4" in src and "k_batch_body<8, 4, 11, rfprog::Prog>" in src
+    assert "rf_batch_w8_h4" in src and "k_batch_body<8, 4, 11, rfprog::BProg>" in src
+    # the batch kernel finds the final xform's terms through the Block
+    assert ("TERM_BEGIN_LOAD>(" in src.split("struct BProg")[1]) == (big.final_xform is not None)
     code, log, sec = rf.program_build(big, 8, 16 + 4, "gfx950")
     assert len(code) > 4096, log
