@@ -360,7 +360,7 @@ int rf_particles_download(rf_ctx* ctx, rf_flame* flame, float* host);
 int rf_rng_download(rf_ctx* ctx, uint32_t* host);
 int rf_shuffle_download(rf_ctx* ctx, uint32_t* host);
 int rf_sample_points_download(rf_ctx* ctx, float* host);
-/* Fast engine: the 16 LDS permutation rows (16 x 64*block_waves u16). */
+/* Fast engine: the 8 LDS permutation rows (8 x 64*block_waves u16). */
 int rf_fast_perms_download(rf_ctx* ctx, uint16_t* host);
 /* Histogram strategy the flame's next fast-engine draw uses (resolves
  * RF_HIST_AUTO) and, if a draw has used RF_HIST_LDS, the fraction of its binned

@@ -691,7 +691,7 @@ void orc_fast_run(const orc_flame* flames, int stride, uint32_t ppt, uint64_t P,
           if (do_draw) cnt += (uint64_t)draw_f32(f, o, &rng, W, H, ss, bins, hits);
         }
       }
-      const uint32_t row = orc_jsf_next_a(&gr) & 15u;
+      const uint32_t row = orc_jsf_next_a(&gr) & (ORC_NPERM - 1u);
       for (uint32_t t = 0; t < N; ++t) {
         const uint32_t d = perms[(size_t)row * N + t];
         nx[3 * d] = px[3 * t]; nx[3 * d + 1] = px[3 * t + 1]; nx[3 * d + 2] = px[3 * t + 2];

@@ -109,7 +109,8 @@ void orc_compat_pass(const orc_flame* flames, int stride, uint32_t T, uint32_t p
 
 /* This project's fast engine schedule (not a reference algorithm): workgroups
  * of 64*waves particles, one xform per wave per iteration, LDS permutation
- * rows from `perms` (16 rows x 64*waves, row = stream & 15). */
+ * rows from `perms` (ORC_NPERM rows x 64*waves, row = stream & (ORC_NPERM - 1)). */
+#define ORC_NPERM 8u
 void orc_fast_run(const orc_flame* flames, int stride, uint32_t ppt, uint64_t P, int waves,
                   const uint16_t* perms, uint64_t seed, uint32_t launch, uint32_t n_iter, int first_run,
                   int do_draw, uint32_t W, uint32_t H, const float ss[6], float* pos,

@@ -507,7 +507,7 @@ class Context:
         return out
 
     def fast_perms(self, block_waves: int = 4) -> np.ndarray:
-        out = np.zeros((16, 64 * block_waves), np.uint16)
+        out = np.zeros((8, 64 * block_waves), np.uint16)
         self.check(self.L.rf_fast_perms_download(self.h, out.ctypes.data_as(C.POINTER(C.c_uint16))))
         return out
 

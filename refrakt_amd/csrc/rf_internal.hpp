@@ -63,7 +63,7 @@ struct DEArgs {
   unsigned long long* done_ts;
 };
 
-constexpr int NPERM = 16;         // permutation rows of the fast engine (held in LDS)
+constexpr int NPERM = 8;          // permutation rows of the fast engine (held in LDS)
 // Draw counters: COUNTER_SLOTS slots of COUNTER_STRIDE u64 (one 128-byte
 // line each; slot s holds [binned samples, LDS-table re-hits]).  Memory-side
 // atomics on one address serialise (about 80 M/s): one counter for every

@@ -122,7 +122,10 @@ std::string program_source(const rfd::Block& b, const std::vector<std::pair<int,
   o << "};\n}  // namespace rfprog\n";
   for (const auto& k : kernels) {
     const int W = k.first, mode = k.second;
-    o << "extern \"C\" __global__ __launch_bounds__(" << 64 * W << ") void " << entry_name(W, mode);
+    o << "extern \"C\" __global__ __launch_bounds__(" << 64 * W << ") ";
+    if (mode < kBatchKernel)
+      o << "__attribute__((amdgpu_waves_per_eu(rfk::detail::fast_waves_per_eu<" << mode << ">()))) ";
+    o << "void " << entry_name(W, mode);
     if (mode >= kBatchKernel) {
       o << "(rfk::BatchArgs g, const rfd::Block* __restrict__ blocks,\n"
         << "    const rfk::BatchFrame* __restrict__ frames) {\n"

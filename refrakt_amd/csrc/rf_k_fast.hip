@@ -13,7 +13,8 @@ namespace {
 using namespace detail;
 
 template <int WAVES, int HM, int LOGK>
-__global__ __launch_bounds__(64 * WAVES) void k_fast(FastArgs g, const Block* __restrict__ blocks) {
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<HM>())))
+void k_fast(FastArgs g, const Block* __restrict__ blocks) {
   k_fast_body<WAVES, HM, LOGK, InterpProg>(g, blocks);
 }
 

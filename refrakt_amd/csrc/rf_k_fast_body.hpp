@@ -35,17 +35,29 @@ using namespace rfd;
 // in registers), wave 0 scans the counts, reserves one run per non-empty
 // bucket with one global atomic and clears the counts, the records are placed
 // in bucket order in LDS and written out so that neighbouring lanes store
-// neighbouring addresses.  Three barriers per flush.
+// neighbouring addresses.  Four barriers per flush.
 template <int N>
 constexpr int sort_stage_iters() { return SORT_STAGE_SLOTS / N; }
+
+// Waves per SIMD the register allocation must allow: every mode but the LDS
+// bin cache (HM 4, two workgroups of 8 waves per CU by its LDS) fits three
+// workgroups of 8 waves per CU in LDS, six waves per SIMD, if each wave
+// keeps to 80 VGPRs.
+template <int HM>
+constexpr int fast_waves_per_eu() { return HM == 4 ? 1 : 6; }
 template <int N>
 constexpr int sort_stage_slots() { return sort_stage_iters<N>() * N; }
 
 template <int N, bool ON>
 struct SortStage {
   static constexpr int S = sort_stage_slots<N>();
-  uint32_t bin[S];       // staged bin index (SORT_NONE: no sample), slot = iteration * N + lane
-  uint32_t rec[S];       // bucket-ordered records
+  // Each lane reads back only its own staged slots, into registers, at the
+  // start of a flush; the bucket-ordered records of the flush then reuse the
+  // memory (the flush ends with a barrier before the next stage writes).
+  union {
+    uint32_t bin[S];     // staged bin index (SORT_NONE: no sample), slot = iteration * N + lane
+    uint32_t rec[S];     // bucket-ordered records
+  };
   uint8_t pid[S];        // staged palette index
   uint8_t bkt[S];        // bucket of rec[j]
   uint32_t cnt[SORT_MAX_BUCKETS];    // records per bucket in this flush (zero between flushes)
@@ -122,8 +134,9 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_st
   }
   lds_barrier();
   // 4. coalesced write-out; records beyond a bucket's capacity go straight to
-  //    the histogram.  Later writes to what this step reads (rec, bkt, base,
-  //    lim, total) come after the next flush's first barrier.
+  //    the histogram.  The barrier at the end keeps the next stage's writes
+  //    (to the slots rec shares) behind every wave's reads; bkt, base, lim
+  //    and total are next written after the next flush's first barrier.
   const uint32_t total = st.total;
   const uint32_t rounds = (total + N - 1) / N;  // uniform
   for (uint32_t r = 0; r < rounds; ++r) {
@@ -143,6 +156,7 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_st
     }
     bin_quad(bins, bin, ovf, pidx, 1.0f, pal);
   }
+  lds_barrier();
 }
 
 template <int WAVES, int HM, int LOGK, class Prog>
@@ -152,14 +166,18 @@ __device__ __forceinline__ void k_fast_body(const FastArgs& g, const Block* __re
   // barrier per iteration separates every write from the reads it must not
   // overtake (the next write to the same buffer is two barriers later).
   __shared__ float s_x[2][N], s_y[2][N], s_c[2][N];
-  __shared__ float4 s_pal[256];
+  // The sort path reads the palette only for the rare records beyond a
+  // bucket's capacity, from global memory: its LDS stays within three
+  // workgroups per CU.
+  __shared__ float4 s_pal[HM == 5 ? 1 : 256];
   // The permutation rows live in LDS: a global load inside the loop would sit
   // behind the iteration's atomics in the in-order vmcnt queue and stall.
   __shared__ uint16_t s_perm[NPERM * N];
   __shared__ LdsBins<LOGK> s_bins;
   __shared__ SortStage<N, HM == 5> s_sort;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palette[i];
+  if constexpr (HM != 5)
+    for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palette[i];
   for (uint32_t i = tid; i < (uint32_t)(NPERM * N); i += N) s_perm[i] = g.perms[i];
   if constexpr (HM == 4) {
     lds_bins_clear(s_bins);
@@ -254,7 +272,7 @@ __device__ __forceinline__ void k_fast_body(const FastArgs& g, const Block* __re
         // the exchange barrier above also orders the staging writes before the flush
         constexpr uint32_t SI = sort_stage_iters<N>();
         if (g.do_draw && ((it % SI) == SI - 1 || it + 1 == g.n_iter))
-          sort_flush(s_sort, (it % SI) + 1, g.sort, g.bins, s_pal);
+          sort_flush(s_sort, (it % SI) + 1, g.sort, g.bins, g.palette);
       }
     }
     g.pos[pid] = make_float4(x, y, c, 0.0f);

@@ -106,7 +106,7 @@ def test_fast_bit_exact_vs_oracle_replica(rf, orc, gpu_available, hist_mode):
     g_hist = hist.download()
     g_pos = fh.particles()
     perms = ctx.fast_perms(4)
-    assert np.array_equal(np.sort(perms, axis=1), np.tile(np.arange(256, dtype=np.uint16), (16, 1)))
+    assert np.array_equal(np.sort(perms, axis=1), np.tile(np.arange(256, dtype=np.uint16), (8, 1)))
 
     fc = orc.flame_to_c(fl)
     ss = orc.screen_affine(fl, W, H)

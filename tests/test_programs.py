@@ -60,7 +60,8 @@ def test_program_source_fixes_the_structure(rf):
     from refrakt_amd import configs
     fl = configs.cfg2().flame
     src = rf.program_source(fl, 8, 5)
-    assert 'extern "C" __global__ __launch_bounds__(512) void rf_prog_w8_h5' in src
+    assert ('extern "C" __global__ __launch_bounds__(512) '
+            '__attribute__((amdgpu_waves_per_eu(rfk::detail::fast_waves_per_eu<5>()))) void rf_prog_w8_h5') in src
     assert "k_fast_body<8, 5, 0, rfprog::Prog>" in src
     # one switch case per xform (the last as default), the final xform inline
     # (once with its term offset fixed, once for the batched renderer)
