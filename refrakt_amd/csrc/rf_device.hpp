@@ -108,19 +108,15 @@ __host__ __device__ __forceinline__ void rf_sincos01(float q, float* s, float* c
 // compiler if-converts it, so every call paid for both paths; in a kernel with
 // ~100 trig call sites that multiplied the iteration's VALU work ~8x.
 //
-// sin/cos: Cody-Waite reduction by 2 pi with a three-part constant (the
-// j * 6.28125 product is exact for |j| < 2^16, so the reduced argument is
-// accurate to a few ulp for |x| < ~4e5), then the hardware v_sin_f32 /
-// v_cos_f32 on the argument in revolutions, [-1/2, 1/2]: max abs error
-// 3.5e-7 on [-pi, pi] (measured against double on MI355X), within the 2^-11
-// GLSL allows for sin/cos.  Seven instructions for both values against about
-// 25 for a polynomial pair with quadrant logic.
+// sin/cos: the argument in revolutions, x / (2 pi) rounded to float, reduced
+// to [0, 1) with v_fract_f32, then the hardware v_sin_f32 / v_cos_f32 (whose
+// own input range ends at 256 revolutions).  The rounding of x / (2 pi) costs
+// at most 2^-25 |x| radians, so the error stays within a few 1e-7 on
+// [-pi, pi] (GLSL asks 2^-11 there and nothing outside it) and grows as
+// 6e-8 |x| beyond.  Four instructions for both values, against eight with
+// a three-part Cody-Waite reduction and about 25 for a polynomial pair.
 __device__ __forceinline__ void rf_sincos(float x, float* s, float* c) {
-  const float j = rintf(x * 0.159154936671257019f);
-  float r = fmaf(-j, 6.28125f, x);
-  r = fmaf(-j, 1.93530716933310032e-3f, r);
-  r = fmaf(-j, 1.02531316770182457e-11f, r);
-  const float t = r * 0.159154936671257019f;
+  const float t = __builtin_amdgcn_fractf(x * 0.159154936671257019f);
   *s = __builtin_amdgcn_sinf(t);
   *c = __builtin_amdgcn_cosf(t);
 }
