@@ -54,12 +54,15 @@ struct SortStage {
   // Each lane reads back only its own staged slots, into registers, at the
   // start of a flush; the bucket-ordered records of the flush then reuse the
   // memory (the flush ends with a barrier before the next stage writes).
+  // rec and bkt have a spare entry S: the placement of an empty slot writes
+  // there, and the write-out reads it for positions past the last record, so
+  // neither needs a branch around its LDS accesses.
   union {
     uint32_t bin[S];     // staged bin index (SORT_NONE: no sample), slot = iteration * N + lane
-    uint32_t rec[S];     // bucket-ordered records
+    uint32_t rec[S + 1]; // bucket-ordered records
   };
   uint8_t pid[S];        // staged palette index
-  uint8_t bkt[S];        // bucket of rec[j]
+  uint8_t bkt[S + 1];    // bucket of rec[j]
   uint32_t cnt[SORT_MAX_BUCKETS];    // records per bucket in this flush (zero between flushes)
   uint32_t off[SORT_MAX_BUCKETS];    // exclusive scan of cnt
   uint32_t base[SORT_MAX_BUCKETS];   // record j of bucket b goes to cbuf[base[b] + j] ...
@@ -122,39 +125,55 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_st
     if (tid == 0) st.total = carry;
   }
   lds_barrier();
-  // 3. place the records in bucket order
+  // 3. place the records in bucket order (all reads first, then the writes;
+  //    an empty slot goes to the spare entry)
+  constexpr uint32_t S = (uint32_t)SortStage<N, true>::S;
+  uint32_t pb[SI], po[SI];
 #pragma unroll
   for (int k = 0; k < SI; ++k) {
-    if (sb[k] != SORT_NONE) {
-      const uint32_t b = sb[k] >> SORT_COARSE_SHIFT;
-      const uint32_t j = st.off[b] + rank[k];
-      st.rec[j] = ((sb[k] & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | st.pid[k * N + tid];
-      st.bkt[j] = (uint8_t)b;
-    }
+    pb[k] = sb[k] != SORT_NONE ? sb[k] >> SORT_COARSE_SHIFT : 0u;
+    po[k] = st.off[pb[k]];
+    rank[k] |= (uint32_t)st.pid[k * N + tid] << 24;  // ranks stay below 2^13
+  }
+#pragma unroll
+  for (int k = 0; k < SI; ++k) {
+    const uint32_t j = sb[k] != SORT_NONE ? po[k] + (rank[k] & 0xffffffu) : S;
+    st.rec[j] = ((sb[k] & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | (rank[k] >> 24);
+    st.bkt[j] = (uint8_t)pb[k];
   }
   lds_barrier();
   // 4. coalesced write-out; records beyond a bucket's capacity go straight to
   //    the histogram.  The barrier at the end keeps the next stage's writes
   //    (to the slots rec shares) behind every wave's reads; bkt, base, lim
   //    and total are next written after the next flush's first barrier.
+  //    Two rounds at a time, their LDS reads issued together; positions past
+  //    the last record read the spare entry and store nothing.
   const uint32_t total = st.total;
   const uint32_t rounds = (total + N - 1) / N;  // uniform
-  for (uint32_t r = 0; r < rounds; ++r) {
-    const uint32_t j = r * N + tid;
-    bool ovf = false;
-    uint32_t bin = 0, pidx = 0;
-    if (j < total) {
-      const uint32_t b = st.bkt[j];
-      const uint32_t rec = st.rec[j];
-      if (j < st.lim[b]) {
-        a.cbuf[st.base[b] + j] = rec;
-      } else {
-        ovf = true;
-        bin = (b << SORT_COARSE_SHIFT) | (rec >> 8);
-        pidx = rec & 255u;
-      }
+  for (uint32_t r = 0; r < rounds; r += 2) {
+    uint32_t j[2], b[2], rc[2], lim[2], base[2];
+    bool in[2], ovf[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      j[h] = (r + (uint32_t)h) * N + tid;
+      in[h] = j[h] < total;
+      const uint32_t jc = in[h] ? j[h] : S;
+      b[h] = st.bkt[jc];
+      rc[h] = st.rec[jc];
     }
-    bin_quad(bins, bin, ovf, pidx, 1.0f, pal);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      lim[h] = st.lim[b[h]];
+      base[h] = st.base[b[h]];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      ovf[h] = in[h] && j[h] >= lim[h];
+      if (in[h] && !ovf[h]) a.cbuf[base[h] + j[h]] = rc[h];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      bin_quad(bins, (b[h] << SORT_COARSE_SHIFT) | (rc[h] >> 8), ovf[h], rc[h] & 255u, 1.0f, pal);
   }
   lds_barrier();
 }

@@ -51,17 +51,27 @@ __global__ __launch_bounds__(RB_THREADS) void k_sort_rebucket(SortArgs a, float*
     s_toff[tid] = a.toff[b * SORT_TILES_PER_BUCKET + tid];
     s_tcap[tid] = a.tcap[b * SORT_TILES_PER_BUCKET + tid];
   }
-  // every thread keeps its RB_PER records in registers between the passes
+  // every thread keeps its RB_PER records in registers between the passes;
+  // a full chunk (all but a bucket's last) issues its loads without per-lane
+  // guards, which would otherwise make each load wait for the one before
   uint32_t rec[RB_PER];
+  if (n == (uint32_t)SORT_CHUNK) {  // workgroup-uniform
 #pragma unroll
-  for (int k = 0; k < RB_PER / 4; ++k) {
-    const uint32_t i4 = (uint32_t)k * RB_THREADS + tid, i = 4u * i4;
-    if (i + 3u < n) {
-      const uint4 q = src4[i4];
+    for (int k = 0; k < RB_PER / 4; ++k) {
+      const uint4 q = src4[(uint32_t)k * RB_THREADS + tid];
       rec[4 * k] = q.x; rec[4 * k + 1] = q.y; rec[4 * k + 2] = q.z; rec[4 * k + 3] = q.w;
-    } else {
+    }
+  } else {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) rec[4 * k + e] = (i + (uint32_t)e) < n ? src[i + e] : SORT_NONE;
+    for (int k = 0; k < RB_PER / 4; ++k) {
+      const uint32_t i4 = (uint32_t)k * RB_THREADS + tid, i = 4u * i4;
+      if (i + 3u < n) {
+        const uint4 q = src4[i4];
+        rec[4 * k] = q.x; rec[4 * k + 1] = q.y; rec[4 * k + 2] = q.z; rec[4 * k + 3] = q.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rec[4 * k + e] = (i + (uint32_t)e) < n ? src[i + e] : SORT_NONE;
+      }
     }
   }
   __syncthreads();
@@ -116,17 +126,39 @@ __global__ __launch_bounds__(RB_THREADS) void k_sort_rebucket(SortArgs a, float*
 constexpr int TILE_THREADS = 512;
 constexpr int TILE_BINS = 1 << SORT_TILE_SHIFT;
 
+struct alignas(16) PalQ {
+  uint64_t rg;  // red | green << 32
+  uint64_t bc;  // blue | 1 << 32
+};
 struct TileSums {
   uint64_t rg[TILE_BINS];  // red | green << 32
   uint64_t bc[TILE_BINS];  // blue | count << 32
-  uint64_t prg[256];       // per palette entry: red | green << 32
-  uint64_t pb[256];        // blue | 1 << 32
+  PalQ pal[256];               // per palette entry, one 16-byte LDS read
 };
 
 __device__ __forceinline__ void tile_add(TileSums& t, uint32_t r) {
   const uint32_t p = r & 255u, i = r >> 8;
-  lds_add_u64(&t.rg[i], t.prg[p]);
-  lds_add_u64(&t.bc[i], t.pb[p]);
+  const PalQ q = t.pal[p];
+  lds_add_u64(&t.rg[i], q.rg);
+  lds_add_u64(&t.bc[i], q.bc);
+}
+
+// Adds K records: every palette read is issued before the first atomic, so a
+// record's atomics do not each wait for their own read.
+template <int K>
+__device__ __forceinline__ void tile_add_batch(TileSums& t, const uint32_t* r) {
+  PalQ v[K];
+  uint32_t bi[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    bi[k] = r[k] >> 8;
+    v[k] = t.pal[r[k] & 255u];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    lds_add_u64(&t.rg[bi[k]], v[k].rg);
+    lds_add_u64(&t.bc[bi[k]], v[k].bc);
+  }
 }
 
 __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(SortArgs a, float4* __restrict__ hist, uint32_t n_bins,
@@ -156,8 +188,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(SortArgs a, float4*
     const float4 c = palette[tid];
     const uint64_t r = (uint32_t)(c.x * one + 0.5f), g = (uint32_t)(c.y * one + 0.5f),
                    bl = (uint32_t)(c.z * one + 0.5f);
-    s_t.prg[tid] = r | (g << 32);
-    s_t.pb[tid] = bl | (1ull << 32);
+    s_t.pal[tid].rg = r | (g << 32);
+    s_t.pal[tid].bc = bl | (1ull << 32);
   }
   __syncthreads();
   // Tile offsets and both segment sizes are multiples of 4 records, so the
@@ -167,18 +199,25 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(SortArgs a, float4*
   const uint32_t n4 = n / 4;
   constexpr uint32_t G = 4;  // 16-byte loads per thread and round
   for (uint32_t i0 = 0; i0 < n4; i0 += G * TILE_THREADS) {
-    uint4 q[G];
+    if (i0 + G * TILE_THREADS <= n4) {  // workgroup-uniform: a full round
+      uint32_t r[4 * G];
 #pragma unroll
-    for (uint32_t k = 0; k < G; ++k) {
-      const uint32_t i = i0 + k * TILE_THREADS + tid;
-      q[k] = i < n4 ? src4[i] : make_uint4(SORT_NONE, SORT_NONE, SORT_NONE, SORT_NONE);
-    }
+      for (uint32_t k = 0; k < G; ++k) {
+        const uint4 q = src4[i0 + k * TILE_THREADS + tid];
+        r[4 * k] = q.x; r[4 * k + 1] = q.y; r[4 * k + 2] = q.z; r[4 * k + 3] = q.w;
+      }
+      tile_add_batch<2 * G>(s_t, r);
+      tile_add_batch<2 * G>(s_t, r + 2 * G);
+    } else {
 #pragma unroll
-    for (uint32_t k = 0; k < G; ++k) {
-      if (q[k].x != SORT_NONE) tile_add(s_t, q[k].x);
-      if (q[k].y != SORT_NONE) tile_add(s_t, q[k].y);
-      if (q[k].z != SORT_NONE) tile_add(s_t, q[k].z);
-      if (q[k].w != SORT_NONE) tile_add(s_t, q[k].w);
+      for (uint32_t k = 0; k < G; ++k) {
+        const uint32_t i = i0 + k * TILE_THREADS + tid;
+        if (i < n4) {
+          const uint4 q = src4[i];
+          const uint32_t r[4] = {q.x, q.y, q.z, q.w};
+          tile_add_batch<4>(s_t, r);
+        }
+      }
     }
   }
   for (uint32_t i = n4 * 4 + tid; i < n; i += TILE_THREADS) tile_add(s_t, src[i]);
