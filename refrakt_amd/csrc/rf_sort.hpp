@@ -19,7 +19,7 @@ constexpr int SORT_TILE_SHIFT = 11;                                          // 
 constexpr int SORT_TILES_PER_BUCKET = 1 << (SORT_COARSE_SHIFT - SORT_TILE_SHIFT);  // 256
 constexpr int SORT_MAX_BUCKETS = 256;                                        // 2^27 bins
 constexpr int SORT_STAGE_SLOTS = 4096;                                       // samples staged in LDS per flush
-constexpr int SORT_CHUNK = 8192;                                             // records per rebucketing workgroup
+constexpr int SORT_CHUNK = 16384;                                            // records per rebucketing workgroup
 // Tile sums add red | green << 32 and blue | count << 32 per record with
 // 64-bit LDS atomics, colours in fixed point.  A tile of at most SORT_SEG16
 // records is summed by one workgroup at 2^16 per unit (a field then holds at
