@@ -115,17 +115,20 @@ __global__ __launch_bounds__(RB_THREADS) void k_sort_rebucket(SortArgs a, float*
   }
 }
 
-// One workgroup per segment of a tile of 2^11 bins (the host lists the
+// One workgroup per segment of a tile of 2^12 bins (the host lists the
 // segments of every tile): the segment's records are summed in LDS with two
 // 64-bit atomics per record, red | green << 32 and blue | count << 32, the
 // colours in fixed point (see SORT_SEG16 and SORT_SEG15 for why no field
 // carries into the next), then the touched bins are added to the histogram:
 // with a plain read-modify-write when the tile is one segment (tiles are
 // disjoint, and every overflow was added by earlier kernels), with quad
-// atomics when several segments share the tile.  36 KiB of LDS: four
-// workgroups per CU.  Each thread loads 16 records (four 16-byte loads in
-// flight) before it issues their LDS atomics.
-constexpr int TILE_THREADS = 512;
+// atomics when several segments share the tile.  68 KiB of LDS: two
+// workgroups of 16 waves per CU (tiles of 2^11 bins, 36 KiB and four
+// workgroups of 8 waves, took 30 % longer on a 1024x1024 draw: the per-tile
+// clearing, palette and write-back are paid over half the records).  Each
+// thread loads 16 records (four 16-byte loads in flight) before it issues
+// their LDS atomics.
+constexpr int TILE_THREADS = 1024;
 constexpr int TILE_BINS = 1 << SORT_TILE_SHIFT;
 
 struct alignas(16) PalQ {

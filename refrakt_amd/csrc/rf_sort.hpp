@@ -2,7 +2,7 @@
 //
 // Instead of one memory-side atomic per sample, the chaos-game kernel writes
 // 4-byte records into coarse buckets of 2^19 bins, a second kernel splits
-// each bucket into 256 tiles of 2^11 bins, and a third accumulates every tile
+// each bucket into 128 tiles of 2^12 bins, and a third accumulates every tile
 // in LDS and adds it to the histogram with one plain read-modify-write per
 // touched bin.  Records carry no weight, so the mode needs unit opacities;
 // the LDS sums are fixed point, so palette values must lie in [0, 1].
@@ -15,8 +15,8 @@
 namespace rfk {
 
 constexpr int SORT_COARSE_SHIFT = 19;                                        // bins per bucket: 2^19
-constexpr int SORT_TILE_SHIFT = 11;                                          // bins per tile: 2^11
-constexpr int SORT_TILES_PER_BUCKET = 1 << (SORT_COARSE_SHIFT - SORT_TILE_SHIFT);  // 256
+constexpr int SORT_TILE_SHIFT = 12;                                          // bins per tile: 2^12
+constexpr int SORT_TILES_PER_BUCKET = 1 << (SORT_COARSE_SHIFT - SORT_TILE_SHIFT);  // 128
 constexpr int SORT_MAX_BUCKETS = 256;                                        // 2^27 bins
 constexpr int SORT_STAGE_SLOTS = 4096;                                       // samples staged in LDS per flush
 constexpr int SORT_CHUNK = 16384;                                            // records per rebucketing workgroup
@@ -33,7 +33,7 @@ constexpr uint32_t SORT_NONE = 0xffffffffu;
 
 struct SortArgs {
   uint32_t* cbuf;          // bucket records: ((bin mod 2^19) << 8) | palette index
-  uint32_t* tbuf;          // tile records:   ((bin mod 2^11) << 8) | palette index
+  uint32_t* tbuf;          // tile records:   ((bin mod 2^12) << 8) | palette index
   uint32_t* bcur;          // per bucket: records offered (may exceed the capacity)
   uint32_t* tcur;          // per tile: records offered
   const uint32_t* boff;    // per bucket: first record in cbuf
