@@ -156,11 +156,14 @@ __device__ __forceinline__ float4_t conv_mfma<32>(const bf16* ap, const bf16* bp
 template <int NT, int KC>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __restrict__ w,
                                                      const float* __restrict__ bias, View out, int H, int W,
-                                                     int Cin, int Cout, int relu) {
+                                                     int Cin, int Cout, int relu, int vec_out) {
   using S = ConvStage<NT, KC>;
   constexpr int PS = S::PS, KE = KC / 4;  // fragment elements per lane
-  __shared__ __attribute__((aligned(16))) bf16 s_in[CT_IN_ROWS * PS];
-  __shared__ __attribute__((aligned(16))) bf16 s_w[9 * 16 * NT * PS];
+  constexpr int OC = 16 * NT, OS = OC + 8;  // output tile: channels per pixel, LDS row stride
+  static_assert(4 * CT_W * OS <= CT_IN_ROWS * PS + 9 * 16 * NT * PS, "output tile fits the staging buffer");
+  __shared__ __attribute__((aligned(16))) bf16 s_buf[CT_IN_ROWS * PS + 9 * 16 * NT * PS];
+  bf16* const s_in = s_buf;
+  bf16* const s_w = s_buf + CT_IN_ROWS * PS;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nrb = (H + CT_R - 1) / CT_R;
   const int n = blockIdx.y / nrb, y0 = (blockIdx.y % nrb) * CT_R;
@@ -202,6 +205,36 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __rest
     }
   }
   const int y = y0 + wave;
+  if (vec_out) {  // workgroup-uniform
+    // Through LDS, so that each pixel's 16*NT output channels leave as 16-byte
+    // stores instead of one 2-byte store per lane and value.
+    __syncthreads();  // the last step's fragment reads are done: the staging buffer is free
+    bf16* const s_o = s_buf + wave * CT_W * OS;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const float bv = bias ? bias[co0 + 16 * t + i] : 0.0f;
+#pragma unroll
+      for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float v = acc[f][t][e] + bv;
+          if (relu) v = fmaxf(v, 0.0f);
+          s_o[(16 * f + 4 * kq + e) * OS + 16 * t + i] = f2bf(v);
+        }
+    }
+    __syncthreads();
+    if (y < H) {
+      constexpr int PQ = OC / 8;  // 16-byte pieces per pixel
+#pragma unroll
+      for (int q = lane; q < CT_W * PQ; q += 64) {
+        const int px = q / PQ, piece = q % PQ;
+        if (x0 + px < W)
+          *reinterpret_cast<uint4*>(out.p + ((size_t)(n * H + y) * W + x0 + px) * out.cs + out.co + co0 + 8 * piece) =
+              *reinterpret_cast<const uint4*>(s_o + px * OS + 8 * piece);
+      }
+    }
+    return;
+  }
   if (y >= H) return;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -477,8 +510,10 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
   if (Cin % 8 == 0 && in.cs % 8 == 0 && in.co % 8 == 0 && ((uintptr_t)in.p & 15u) == 0 &&
       ((uintptr_t)w & 15u) == 0) {
     const dim3 grid((W + CT_W - 1) / CT_W, N * ((H + CT_R - 1) / CT_R), Cout / (16 * nt));
-#define RF_CONV_LDS(NT_, KC_) \
-  hipLaunchKernelGGL((k_conv3x3_lds<NT_, KC_>), grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu)
+    const int vec_out = out.cs % 8 == 0 && out.co % 8 == 0 && ((uintptr_t)out.p & 15u) == 0;
+#define RF_CONV_LDS(NT_, KC_)                                                                                  \
+  hipLaunchKernelGGL((k_conv3x3_lds<NT_, KC_>), grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu, \
+                     vec_out)
     // K = 32 steps pay off where each staged step feeds 64 output channels;
     // the narrow full-resolution layers keep the smaller K = 16 tiles, whose
     // LDS and registers fit four workgroups per CU (measured: 32-channel
