@@ -99,14 +99,25 @@ int check_shape(rf_denoiser* d, int N, int H, int W) {
   return RF_OK;
 }
 
-// Forward pass on x (N x H x W x 16 bf16).
-int forward(rf_denoiser* d, const bf16* x, int N, int H, int W) {
+// Forward pass on x (N x H x W x 16 bf16).  `fused_up`: the decoder's
+// convolutions read the upsampled half of their concatenated input straight
+// from the lower level (inference); training writes it into the concatenation
+// buffers, whose full contents the weight gradients read back.
+int forward(rf_denoiser* d, const bf16* x, int N, int H, int W, bool fused_up) {
   rf_ctx* ctx = d->ctx;
   hipStream_t s = ctx->stream;
   const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
   const View vx{(bf16*)x, 16, 0};
   auto conv = [&](int l, View in, View out, int h, int w) {
     return rfn::conv3x3(in, w_of(d, l), bias_of(d, l), out, N, h, w, d->L[l].cin, d->L[l].cout, d->L[l].relu, s);
+  };
+  // conv l over [upsample(lo) (c_lo channels), the skip channels of `cat` from c_lo on]
+  auto up_conv = [&](int l, View lo, int c_lo, DevBuf& cat, int cat_cs, View out, int h, int w) -> hipError_t {
+    if (fused_up)
+      return rfn::conv3x3(V(cat, cat_cs, c_lo), w_of(d, l), bias_of(d, l), out, N, h, w, d->L[l].cin, d->L[l].cout,
+                          d->L[l].relu, s, lo, c_lo);
+    const hipError_t e = rfn::upsample2(lo, V(cat, cat_cs, 0), N, h / 2, w / 2, c_lo, s);
+    return e != hipSuccess ? e : conv(l, V(cat, cat_cs), out, h, w);
   };
   HIPCHK(ctx, conv(0, vx, V(d->e0a, 32), H, W));
   HIPCHK(ctx, conv(1, V(d->e0a, 32), V(d->cat0, 96, 64), H, W));
@@ -118,12 +129,9 @@ int forward(rf_denoiser* d, const bf16* x, int N, int H, int W) {
   HIPCHK(ctx, conv(5, V(d->e2a, 128), V(d->cat2, 384, 256), H2, W2));
   HIPCHK(ctx, rfn::avgpool2(V(d->cat2, 384, 256), V(d->p2, 128), N, H2, W2, 128, s));
   HIPCHK(ctx, conv(6, V(d->p2, 128), V(d->b, 256), H3, W3));
-  HIPCHK(ctx, rfn::upsample2(V(d->b, 256), V(d->cat2, 384, 0), N, H3, W3, 256, s));
-  HIPCHK(ctx, conv(7, V(d->cat2, 384), V(d->d2, 128), H2, W2));
-  HIPCHK(ctx, rfn::upsample2(V(d->d2, 128), V(d->cat1, 192, 0), N, H2, W2, 128, s));
-  HIPCHK(ctx, conv(8, V(d->cat1, 192), V(d->d1, 64), H1, W1));
-  HIPCHK(ctx, rfn::upsample2(V(d->d1, 64), V(d->cat0, 96, 0), N, H1, W1, 64, s));
-  HIPCHK(ctx, conv(9, V(d->cat0, 96), V(d->d0a, 32), H, W));
+  HIPCHK(ctx, up_conv(7, V(d->b, 256), 256, d->cat2, 384, V(d->d2, 128), H2, W2));
+  HIPCHK(ctx, up_conv(8, V(d->d2, 128), 128, d->cat1, 192, V(d->d1, 64), H1, W1));
+  HIPCHK(ctx, up_conv(9, V(d->d1, 64), 64, d->cat0, 96, V(d->d0a, 32), H, W));
   HIPCHK(ctx, conv(10, V(d->d0a, 32), V(d->d0b, 32), H, W));
   HIPCHK(ctx, conv(11, V(d->d0b, 32), V(d->o, 16), H, W));
   return RF_OK;
@@ -274,7 +282,7 @@ int rf_denoiser_train_step(rf_denoiser* d, const void* x, const void* target, in
   if ((rc = ensure_shape(d, N, H, W))) return rc;
   rf_ctx* ctx = d->ctx;
   hipStream_t s = ctx->stream;
-  if ((rc = forward(d, (const bf16*)x, N, H, W))) return rc;
+  if ((rc = forward(d, (const bf16*)x, N, H, W, false))) return rc;
   const int npix = N * H * W;
   HIPCHK(ctx, rfn::zero_f32((float*)d->loss.p, 1, s));
   HIPCHK(ctx, rfn::residual_loss(View{(bf16*)x, 16, 0}, V(d->o, 16), (const bf16*)target, nullptr,
@@ -301,7 +309,7 @@ int rf_denoiser_infer(rf_denoiser* d, const void* x, float* y, int32_t N, int32_
   int rc = check_shape(d, N, H, W);
   if (rc) return rc;
   if ((rc = ensure_shape(d, N, H, W))) return rc;
-  if ((rc = forward(d, (const bf16*)x, N, H, W))) return rc;
+  if ((rc = forward(d, (const bf16*)x, N, H, W, true))) return rc;
   HIPCHK(d->ctx, rfn::residual_loss(View{(bf16*)x, 16, 0}, V(d->o, 16), nullptr, y, nullptr, nullptr, N * H * W,
                                     0.0f, d->ctx->stream));
   HIPCHK(d->ctx, hipStreamSynchronize(d->ctx->stream));
@@ -316,7 +324,7 @@ int rf_denoiser_grads(rf_denoiser* d, const void* x, const void* target, int32_t
   if ((rc = ensure_shape(d, N, H, W))) return rc;
   rf_ctx* ctx = d->ctx;
   hipStream_t s = ctx->stream;
-  if ((rc = forward(d, (const bf16*)x, N, H, W))) return rc;
+  if ((rc = forward(d, (const bf16*)x, N, H, W, false))) return rc;
   const int npix = N * H * W;
   HIPCHK(ctx, rfn::zero_f32((float*)d->loss.p, 1, s));
   HIPCHK(ctx, rfn::residual_loss(View{(bf16*)x, 16, 0}, V(d->o, 16), (const bf16*)target, nullptr,

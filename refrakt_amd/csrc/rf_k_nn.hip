@@ -110,11 +110,16 @@ struct ConvStage {
   uint4 vw[W_PER];
 };
 
+// Channels [0, Cup) come from the half-resolution `up` (nearest-neighbour
+// upsampling read in place), the rest from `in`; a KC step never straddles
+// the two (Cup is a multiple of 32).
 template <int NT, int KC>
-__device__ __forceinline__ void conv_load(ConvStage<NT, KC>& st, const View& in, const bf16* __restrict__ w, int n,
-                                          int y0, int x0, int co0, int c0, int H, int W, int Cin, int Cout) {
+__device__ __forceinline__ void conv_load(ConvStage<NT, KC>& st, const View& in, const View& up, int Cup,
+                                          const bf16* __restrict__ w, int n, int y0, int x0, int co0, int c0, int H,
+                                          int W, int Cin, int Cout) {
   using S = ConvStage<NT, KC>;
   const int tid = threadIdx.x;
+  const bool from_up = c0 < Cup;  // workgroup-uniform
 #pragma unroll
   for (int k = 0; k < S::IN_PER; ++k) {
     const int q = tid + 256 * k;
@@ -122,8 +127,12 @@ __device__ __forceinline__ void conv_load(ConvStage<NT, KC>& st, const View& in,
     if (q < S::IN_Q) {
       const int piece = q % S::PQ, pix = q / S::PQ;
       const int yy = y0 + pix / (CT_W + 2) - 1, gx = x0 + pix % (CT_W + 2) - 1;
-      if (yy >= 0 && yy < H && gx >= 0 && gx < W)
-        v = *reinterpret_cast<const uint4*>(in.p + ((size_t)(n * H + yy) * W + gx) * in.cs + in.co + c0 + 8 * piece);
+      if (yy >= 0 && yy < H && gx >= 0 && gx < W) {
+        const bf16* src = from_up ? up.p + ((size_t)(n * (H >> 1) + (yy >> 1)) * (W >> 1) + (gx >> 1)) * up.cs +
+                                        up.co + c0 + 8 * piece
+                                  : in.p + ((size_t)(n * H + yy) * W + gx) * in.cs + in.co + (c0 - Cup) + 8 * piece;
+        v = *reinterpret_cast<const uint4*>(src);
+      }
     }
     st.vin[k] = v;
   }
@@ -156,7 +165,7 @@ __device__ __forceinline__ float4_t conv_mfma<32>(const bf16* ap, const bf16* bp
 template <int NT, int KC>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __restrict__ w,
                                                      const float* __restrict__ bias, View out, int H, int W,
-                                                     int Cin, int Cout, int relu, int vec_out) {
+                                                     int Cin, int Cout, int relu, int vec_out, View up, int Cup) {
   using S = ConvStage<NT, KC>;
   constexpr int PS = S::PS, KE = KC / 4;  // fragment elements per lane
   constexpr int OC = 16 * NT, OS = OC + 8;  // output tile: channels per pixel, LDS row stride
@@ -176,7 +185,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __rest
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[f][t] = float4_t{0.f, 0.f, 0.f, 0.f};
   S st;
-  conv_load<NT, KC>(st, in, w, n, y0, x0, co0, 0, H, W, Cin, Cout);
+  conv_load<NT, KC>(st, in, up, Cup, w, n, y0, x0, co0, 0, H, W, Cin, Cout);
   for (int c0 = 0; c0 < Cin; c0 += KC) {
     __syncthreads();  // the previous step's fragment reads are done
 #pragma unroll
@@ -190,7 +199,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __rest
       if (q < S::W_Q) *reinterpret_cast<uint4*>(s_w + (q / S::PQ) * PS + 8 * (q % S::PQ)) = st.vw[k];
     }
     __syncthreads();
-    if (c0 + KC < Cin) conv_load<NT, KC>(st, in, w, n, y0, x0, co0, c0 + KC, H, W, Cin, Cout);
+    if (c0 + KC < Cin) conv_load<NT, KC>(st, in, up, Cup, w, n, y0, x0, co0, c0 + KC, H, W, Cin, Cout);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
@@ -503,17 +512,23 @@ inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 }  // namespace
 
 hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, int H, int W, int Cin, int Cout,
-                   int relu, hipStream_t s) {
-  if (Cin % 16 || Cout % 16 || in.cs % 4 || in.co % 4 || out.cs < out.co + Cout || in.cs < in.co + Cin)
+                   int relu, hipStream_t s, View up, int Cup) {
+  if (Cin % 16 || Cout % 16 || in.cs % 4 || in.co % 4 || out.cs < out.co + Cout || in.cs < in.co + Cin - Cup)
     return hipErrorInvalidValue;
   const int nt = (Cout % 64 == 0) ? 4 : (Cout % 32 == 0) ? 2 : 1;
-  if (Cin % 8 == 0 && in.cs % 8 == 0 && in.co % 8 == 0 && ((uintptr_t)in.p & 15u) == 0 &&
-      ((uintptr_t)w & 15u) == 0) {
+  const bool lds = Cin % 8 == 0 && in.cs % 8 == 0 && in.co % 8 == 0 && ((uintptr_t)in.p & 15u) == 0 &&
+                   ((uintptr_t)w & 15u) == 0;
+  if (Cup) {  // the fused upsampling exists on the LDS path only
+    if (!lds || Cup < 0 || Cup % 32 || Cup >= Cin || H % 2 || W % 2 || up.cs % 8 || up.co % 8 ||
+        up.cs < up.co + Cup || ((uintptr_t)up.p & 15u) != 0)
+      return hipErrorInvalidValue;
+  }
+  if (lds) {
     const dim3 grid((W + CT_W - 1) / CT_W, N * ((H + CT_R - 1) / CT_R), Cout / (16 * nt));
     const int vec_out = out.cs % 8 == 0 && out.co % 8 == 0 && ((uintptr_t)out.p & 15u) == 0;
 #define RF_CONV_LDS(NT_, KC_)                                                                                  \
   hipLaunchKernelGGL((k_conv3x3_lds<NT_, KC_>), grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu, \
-                     vec_out)
+                     vec_out, up, Cup)
     // K = 32 steps pay off where each staged step feeds 64 output channels;
     // the narrow full-resolution layers keep the smaller K = 16 tiles, whose
     // LDS and registers fit four workgroups per CU (measured: 32-channel
