@@ -111,6 +111,11 @@ int forward(rf_denoiser* d, const bf16* x, int N, int H, int W, bool fused_up) {
   auto conv = [&](int l, View in, View out, int h, int w) {
     return rfn::conv3x3(in, w_of(d, l), bias_of(d, l), out, N, h, w, d->L[l].cin, d->L[l].cout, d->L[l].relu, s);
   };
+  // conv l that also writes the 2x2 average of its output to `pool`
+  auto conv_pool = [&](int l, View in, View out, View pool, int h, int w) {
+    return rfn::conv3x3(in, w_of(d, l), bias_of(d, l), out, N, h, w, d->L[l].cin, d->L[l].cout, d->L[l].relu, s,
+                        View{nullptr, 0, 0}, 0, pool);
+  };
   // conv l over [upsample(lo) (c_lo channels), the skip channels of `cat` from c_lo on]
   auto up_conv = [&](int l, View lo, int c_lo, DevBuf& cat, int cat_cs, View out, int h, int w) -> hipError_t {
     if (fused_up)
@@ -120,14 +125,11 @@ int forward(rf_denoiser* d, const bf16* x, int N, int H, int W, bool fused_up) {
     return e != hipSuccess ? e : conv(l, V(cat, cat_cs), out, h, w);
   };
   HIPCHK(ctx, conv(0, vx, V(d->e0a, 32), H, W));
-  HIPCHK(ctx, conv(1, V(d->e0a, 32), V(d->cat0, 96, 64), H, W));
-  HIPCHK(ctx, rfn::avgpool2(V(d->cat0, 96, 64), V(d->p0, 32), N, H, W, 32, s));
+  HIPCHK(ctx, conv_pool(1, V(d->e0a, 32), V(d->cat0, 96, 64), V(d->p0, 32), H, W));
   HIPCHK(ctx, conv(2, V(d->p0, 32), V(d->e1a, 64), H1, W1));
-  HIPCHK(ctx, conv(3, V(d->e1a, 64), V(d->cat1, 192, 128), H1, W1));
-  HIPCHK(ctx, rfn::avgpool2(V(d->cat1, 192, 128), V(d->p1, 64), N, H1, W1, 64, s));
+  HIPCHK(ctx, conv_pool(3, V(d->e1a, 64), V(d->cat1, 192, 128), V(d->p1, 64), H1, W1));
   HIPCHK(ctx, conv(4, V(d->p1, 64), V(d->e2a, 128), H2, W2));
-  HIPCHK(ctx, conv(5, V(d->e2a, 128), V(d->cat2, 384, 256), H2, W2));
-  HIPCHK(ctx, rfn::avgpool2(V(d->cat2, 384, 256), V(d->p2, 128), N, H2, W2, 128, s));
+  HIPCHK(ctx, conv_pool(5, V(d->e2a, 128), V(d->cat2, 384, 256), V(d->p2, 128), H2, W2));
   HIPCHK(ctx, conv(6, V(d->p2, 128), V(d->b, 256), H3, W3));
   HIPCHK(ctx, up_conv(7, V(d->b, 256), 256, d->cat2, 384, V(d->d2, 128), H2, W2));
   HIPCHK(ctx, up_conv(8, V(d->d2, 128), 128, d->cat1, 192, V(d->d1, 64), H1, W1));

@@ -79,6 +79,9 @@ __global__ __launch_bounds__(256) void k_conv3x3(View in, const bf16* __restrict
   }
 }
 
+__device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
+
 // LDS-tiled implicit GEMM.  A workgroup of 4 waves covers CT_R = 4 image rows
 // x CT_W = 64 pixels (one row per wave, four 16-pixel A fragments) and 16*NT
 // output channels.  Per KC-channel step of Cin (KC = 32 with gfx950's
@@ -165,7 +168,8 @@ __device__ __forceinline__ float4_t conv_mfma<32>(const bf16* ap, const bf16* bp
 template <int NT, int KC>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __restrict__ w,
                                                      const float* __restrict__ bias, View out, int H, int W,
-                                                     int Cin, int Cout, int relu, int vec_out, View up, int Cup) {
+                                                     int Cin, int Cout, int relu, int vec_out, View up, int Cup,
+                                                     View pool) {
   using S = ConvStage<NT, KC>;
   constexpr int PS = S::PS, KE = KC / 4;  // fragment elements per lane
   constexpr int OC = 16 * NT, OS = OC + 8;  // output tile: channels per pixel, LDS row stride
@@ -232,14 +236,40 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __rest
         }
     }
     __syncthreads();
+    constexpr int PQ = OC / 8;  // 16-byte pieces per pixel
     if (y < H) {
-      constexpr int PQ = OC / 8;  // 16-byte pieces per pixel
 #pragma unroll
       for (int q = lane; q < CT_W * PQ; q += 64) {
         const int px = q / PQ, piece = q % PQ;
         if (x0 + px < W)
           *reinterpret_cast<uint4*>(out.p + ((size_t)(n * H + y) * W + x0 + px) * out.cs + out.co + co0 + 8 * piece) =
               *reinterpret_cast<const uint4*>(s_o + px * OS + 8 * piece);
+      }
+    }
+    if (pool.p) {  // workgroup-uniform: the tile's 2x2 averages, summed in k_avgpool2_v8's order
+      constexpr int PW = CT_W / 2, NPQ = (CT_R / 2) * PW * PQ;
+      for (int q = tid; q < NPQ; q += 256) {
+        const int piece = q % PQ, pp = (q / PQ) % PW, pr = q / (PQ * PW);
+        const int yo = (y0 >> 1) + pr, xo = (x0 >> 1) + pp;
+        if (yo < (H >> 1) && xo < (W >> 1)) {
+          float sum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              const uint4 v =
+                  *reinterpret_cast<const uint4*>(s_buf + (2 * pr + a) * CT_W * OS + (2 * pp + b) * OS + 8 * piece);
+              const uint32_t qv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+              for (int k = 0; k < 4; ++k) { sum[2 * k] += bf_lo(qv[k]); sum[2 * k + 1] += bf_hi(qv[k]); }
+            }
+          uint32_t r[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            r[k] = (uint32_t)f2bf(0.25f * sum[2 * k]) | ((uint32_t)f2bf(0.25f * sum[2 * k + 1]) << 16);
+          *reinterpret_cast<uint4*>(pool.p + ((size_t)(n * (H >> 1) + yo) * (W >> 1) + xo) * pool.cs + pool.co +
+                                    co0 + 8 * piece) = make_uint4(r[0], r[1], r[2], r[3]);
+        }
       }
     }
     return;
@@ -382,8 +412,6 @@ __global__ void k_upsample2(View in, View out, int N, int H, int W, int C) {
 
 // 8-channel (16-byte) versions of pooling and upsampling: one thread per
 // output pixel and channel group.
-__device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
-__device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
 __global__ void k_avgpool2_v8(View in, View out, int N, int H, int W, int C) {
   const int Ho = H / 2, Wo = W / 2, G = C / 8;
@@ -512,7 +540,7 @@ inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 }  // namespace
 
 hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, int H, int W, int Cin, int Cout,
-                   int relu, hipStream_t s, View up, int Cup) {
+                   int relu, hipStream_t s, View up, int Cup, View pool) {
   if (Cin % 16 || Cout % 16 || in.cs % 4 || in.co % 4 || out.cs < out.co + Cout || in.cs < in.co + Cin - Cup)
     return hipErrorInvalidValue;
   const int nt = (Cout % 64 == 0) ? 4 : (Cout % 32 == 0) ? 2 : 1;
@@ -526,9 +554,13 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
   if (lds) {
     const dim3 grid((W + CT_W - 1) / CT_W, N * ((H + CT_R - 1) / CT_R), Cout / (16 * nt));
     const int vec_out = out.cs % 8 == 0 && out.co % 8 == 0 && ((uintptr_t)out.p & 15u) == 0;
+    // the fused pooling needs the LDS epilogue; otherwise pool afterwards
+    const bool fuse_pool = pool.p && vec_out && H % 2 == 0 && W % 2 == 0 && pool.cs % 8 == 0 &&
+                           pool.co % 8 == 0 && ((uintptr_t)pool.p & 15u) == 0;
+    const View pk = fuse_pool ? pool : View{nullptr, 0, 0};
 #define RF_CONV_LDS(NT_, KC_)                                                                                  \
   hipLaunchKernelGGL((k_conv3x3_lds<NT_, KC_>), grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu, \
-                     vec_out, up, Cup)
+                     vec_out, up, Cup, pk)
     // K = 32 steps pay off where each staged step feeds 64 output channels;
     // the narrow full-resolution layers keep the smaller K = 16 tiles, whose
     // LDS and registers fit four workgroups per CU (measured: 32-channel
@@ -543,7 +575,9 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
       }
     }
 #undef RF_CONV_LDS
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !pool.p || fuse_pool) return e;
+    return avgpool2(out, pool, N, H, W, Cout, s);
   }
   const dim3 grid((W + 63) / 64, N * H, Cout / (16 * nt));
   switch (nt) {
@@ -551,7 +585,9 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
     case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
     default: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
   }
-  return hipGetLastError();
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !pool.p) return e;
+  return avgpool2(out, pool, N, H, W, Cout, s);
 }
 
 hipError_t conv3x3_wgrad(View x, View dy, float* dw, float* db, int N, int H, int W, int Cin, int Cout,

@@ -25,9 +25,11 @@ struct View {
 // With Cup > 0 the first Cup input channels are the nearest-neighbour 2x
 // upsampling of `up` (an N x H/2 x W/2 tensor), read in place, and `in`
 // holds channels Cup..Cin-1 (Cup a multiple of 32, H and W even, 16-byte
-// aligned channel runs in both views).
+// aligned channel runs in both views).  With pool.p set, the 2x2 average
+// of the output (as avgpool2 computes it) is also written to `pool`.
 hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, int H, int W, int Cin, int Cout,
-                   int relu, hipStream_t s, View up = View{nullptr, 0, 0}, int Cup = 0);
+                   int relu, hipStream_t s, View up = View{nullptr, 0, 0}, int Cup = 0,
+                   View pool = View{nullptr, 0, 0});
 // Weight gradient: dw[r][s][k][c] += sum_p dy[p,k] * x[p + (r-1, s-1), c] and
 // db[k] += sum_p dy[p,k] (fp32, accumulated with atomics; zero them first).
 hipError_t conv3x3_wgrad(View x, View dy, float* dw, float* db, int N, int H, int W, int Cin, int Cout,
