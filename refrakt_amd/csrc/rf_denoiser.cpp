@@ -99,11 +99,14 @@ int check_shape(rf_denoiser* d, int N, int H, int W) {
   return RF_OK;
 }
 
-// Forward pass on x (N x H x W x 16 bf16).  `fused_up`: the decoder's
-// convolutions read the upsampled half of their concatenated input straight
-// from the lower level (inference); training writes it into the concatenation
-// buffers, whose full contents the weight gradients read back.
-int forward(rf_denoiser* d, const bf16* x, int N, int H, int W, bool fused_up) {
+// Forward pass on x (N x H x W x 16 bf16).  With y_out (inference) the
+// decoder's convolutions read the upsampled half of their concatenated input
+// straight from the lower level and the last one writes y_out = x[:4] +
+// out[:4] (fp32); training writes the upsampled halves into the
+// concatenation buffers, whose full contents the weight gradients read back,
+// and keeps the last layer's output for the loss.
+int forward(rf_denoiser* d, const bf16* x, int N, int H, int W, float* y_out = nullptr) {
+  const bool fused_up = y_out != nullptr;
   rf_ctx* ctx = d->ctx;
   hipStream_t s = ctx->stream;
   const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
@@ -112,15 +115,22 @@ int forward(rf_denoiser* d, const bf16* x, int N, int H, int W, bool fused_up) {
     return rfn::conv3x3(in, w_of(d, l), bias_of(d, l), out, N, h, w, d->L[l].cin, d->L[l].cout, d->L[l].relu, s);
   };
   // conv l that also writes the 2x2 average of its output to `pool`
+  auto conv_f = [&](int l, View in, View out, int h, int w, const rfn::ConvFuse& f) {
+    return rfn::conv3x3(in, w_of(d, l), bias_of(d, l), out, N, h, w, d->L[l].cin, d->L[l].cout, d->L[l].relu, s, f);
+  };
   auto conv_pool = [&](int l, View in, View out, View pool, int h, int w) {
-    return rfn::conv3x3(in, w_of(d, l), bias_of(d, l), out, N, h, w, d->L[l].cin, d->L[l].cout, d->L[l].relu, s,
-                        View{nullptr, 0, 0}, 0, pool);
+    rfn::ConvFuse f;
+    f.pool = pool;
+    return conv_f(l, in, out, h, w, f);
   };
   // conv l over [upsample(lo) (c_lo channels), the skip channels of `cat` from c_lo on]
   auto up_conv = [&](int l, View lo, int c_lo, DevBuf& cat, int cat_cs, View out, int h, int w) -> hipError_t {
-    if (fused_up)
-      return rfn::conv3x3(V(cat, cat_cs, c_lo), w_of(d, l), bias_of(d, l), out, N, h, w, d->L[l].cin, d->L[l].cout,
-                          d->L[l].relu, s, lo, c_lo);
+    if (fused_up) {
+      rfn::ConvFuse f;
+      f.up = lo;
+      f.Cup = c_lo;
+      return conv_f(l, V(cat, cat_cs, c_lo), out, h, w, f);
+    }
     const hipError_t e = rfn::upsample2(lo, V(cat, cat_cs, 0), N, h / 2, w / 2, c_lo, s);
     return e != hipSuccess ? e : conv(l, V(cat, cat_cs), out, h, w);
   };
@@ -135,7 +145,14 @@ int forward(rf_denoiser* d, const bf16* x, int N, int H, int W, bool fused_up) {
   HIPCHK(ctx, up_conv(8, V(d->d2, 128), 128, d->cat1, 192, V(d->d1, 64), H1, W1));
   HIPCHK(ctx, up_conv(9, V(d->d1, 64), 64, d->cat0, 96, V(d->d0a, 32), H, W));
   HIPCHK(ctx, conv(10, V(d->d0a, 32), V(d->d0b, 32), H, W));
-  HIPCHK(ctx, conv(11, V(d->d0b, 32), V(d->o, 16), H, W));
+  if (y_out) {
+    rfn::ConvFuse f;
+    f.res = vx;
+    f.y = y_out;
+    HIPCHK(ctx, conv_f(11, V(d->d0b, 32), V(d->o, 16), H, W, f));
+  } else {
+    HIPCHK(ctx, conv(11, V(d->d0b, 32), V(d->o, 16), H, W));
+  }
   return RF_OK;
 }
 
@@ -284,7 +301,7 @@ int rf_denoiser_train_step(rf_denoiser* d, const void* x, const void* target, in
   if ((rc = ensure_shape(d, N, H, W))) return rc;
   rf_ctx* ctx = d->ctx;
   hipStream_t s = ctx->stream;
-  if ((rc = forward(d, (const bf16*)x, N, H, W, false))) return rc;
+  if ((rc = forward(d, (const bf16*)x, N, H, W))) return rc;
   const int npix = N * H * W;
   HIPCHK(ctx, rfn::zero_f32((float*)d->loss.p, 1, s));
   HIPCHK(ctx, rfn::residual_loss(View{(bf16*)x, 16, 0}, V(d->o, 16), (const bf16*)target, nullptr,
@@ -311,9 +328,7 @@ int rf_denoiser_infer(rf_denoiser* d, const void* x, float* y, int32_t N, int32_
   int rc = check_shape(d, N, H, W);
   if (rc) return rc;
   if ((rc = ensure_shape(d, N, H, W))) return rc;
-  if ((rc = forward(d, (const bf16*)x, N, H, W, true))) return rc;
-  HIPCHK(d->ctx, rfn::residual_loss(View{(bf16*)x, 16, 0}, V(d->o, 16), nullptr, y, nullptr, nullptr, N * H * W,
-                                    0.0f, d->ctx->stream));
+  if ((rc = forward(d, (const bf16*)x, N, H, W, y))) return rc;
   HIPCHK(d->ctx, hipStreamSynchronize(d->ctx->stream));
   return RF_OK;
 }
@@ -326,7 +341,7 @@ int rf_denoiser_grads(rf_denoiser* d, const void* x, const void* target, int32_t
   if ((rc = ensure_shape(d, N, H, W))) return rc;
   rf_ctx* ctx = d->ctx;
   hipStream_t s = ctx->stream;
-  if ((rc = forward(d, (const bf16*)x, N, H, W, false))) return rc;
+  if ((rc = forward(d, (const bf16*)x, N, H, W))) return rc;
   const int npix = N * H * W;
   HIPCHK(ctx, rfn::zero_f32((float*)d->loss.p, 1, s));
   HIPCHK(ctx, rfn::residual_loss(View{(bf16*)x, 16, 0}, V(d->o, 16), (const bf16*)target, nullptr,

@@ -168,8 +168,10 @@ __device__ __forceinline__ float4_t conv_mfma<32>(const bf16* ap, const bf16* bp
 template <int NT, int KC>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __restrict__ w,
                                                      const float* __restrict__ bias, View out, int H, int W,
-                                                     int Cin, int Cout, int relu, int vec_out, View up, int Cup,
-                                                     View pool) {
+                                                     int Cin, int Cout, int relu, int vec_out, ConvFuse fz) {
+  const View& up = fz.up;
+  const int Cup = fz.Cup;
+  const View& pool = fz.pool;
   using S = ConvStage<NT, KC>;
   constexpr int PS = S::PS, KE = KC / 4;  // fragment elements per lane
   constexpr int OC = 16 * NT, OS = OC + 8;  // output tile: channels per pixel, LDS row stride
@@ -237,6 +239,18 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __rest
     }
     __syncthreads();
     constexpr int PQ = OC / 8;  // 16-byte pieces per pixel
+    if (fz.y) {  // workgroup-uniform: the residual output replaces `out`
+      const int px = x0 + lane;
+      if (y < H && px < W) {
+        const size_t p = (size_t)(n * H + y) * W + px;
+        const uint2 xv = *reinterpret_cast<const uint2*>(fz.res.p + p * fz.res.cs + fz.res.co);
+        const uint2 ov = *reinterpret_cast<const uint2*>(s_o + lane * OS);
+        *reinterpret_cast<float4*>(fz.y + p * 4) =
+            make_float4(bf_lo(xv.x) + bf_lo(ov.x), bf_hi(xv.x) + bf_hi(ov.x), bf_lo(xv.y) + bf_lo(ov.y),
+                        bf_hi(xv.y) + bf_hi(ov.y));
+      }
+      return;
+    }
     if (y < H) {
 #pragma unroll
       for (int q = lane; q < CT_W * PQ; q += 64) {
@@ -540,7 +554,10 @@ inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 }  // namespace
 
 hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, int H, int W, int Cin, int Cout,
-                   int relu, hipStream_t s, View up, int Cup, View pool) {
+                   int relu, hipStream_t s, const ConvFuse& fuse) {
+  const View& up = fuse.up;
+  const int Cup = fuse.Cup;
+  const View& pool = fuse.pool;
   if (Cin % 16 || Cout % 16 || in.cs % 4 || in.co % 4 || out.cs < out.co + Cout || in.cs < in.co + Cin - Cup)
     return hipErrorInvalidValue;
   const int nt = (Cout % 64 == 0) ? 4 : (Cout % 32 == 0) ? 2 : 1;
@@ -557,10 +574,16 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
     // the fused pooling needs the LDS epilogue; otherwise pool afterwards
     const bool fuse_pool = pool.p && vec_out && H % 2 == 0 && W % 2 == 0 && pool.cs % 8 == 0 &&
                            pool.co % 8 == 0 && ((uintptr_t)pool.p & 15u) == 0;
-    const View pk = fuse_pool ? pool : View{nullptr, 0, 0};
+    // the residual output needs the LDS epilogue of a single 16-channel block;
+    // otherwise the output is written and the residual computed afterwards
+    const bool fuse_res = fuse.y && vec_out && Cout == 16 && fuse.res.cs % 4 == 0 && fuse.res.co % 4 == 0 &&
+                          ((uintptr_t)fuse.res.p & 7u) == 0 && ((uintptr_t)fuse.y & 15u) == 0 && !pool.p;
+    ConvFuse fz = fuse;
+    if (!fuse_pool) fz.pool = View{nullptr, 0, 0};
+    if (!fuse_res) { fz.res = View{nullptr, 0, 0}; fz.y = nullptr; }
 #define RF_CONV_LDS(NT_, KC_)                                                                                  \
   hipLaunchKernelGGL((k_conv3x3_lds<NT_, KC_>), grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu, \
-                     vec_out, up, Cup, pk)
+                     vec_out, fz)
     // K = 32 steps pay off where each staged step feeds 64 output channels;
     // the narrow full-resolution layers keep the smaller K = 16 tiles, whose
     // LDS and registers fit four workgroups per CU (measured: 32-channel
@@ -575,9 +598,11 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
       }
     }
 #undef RF_CONV_LDS
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !pool.p || fuse_pool) return e;
-    return avgpool2(out, pool, N, H, W, Cout, s);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && pool.p && !fuse_pool) e = avgpool2(out, pool, N, H, W, Cout, s);
+    if (e == hipSuccess && fuse.y && !fuse_res)
+      e = residual_loss(fuse.res, out, nullptr, fuse.y, nullptr, nullptr, N * H * W, 0.0f, s);
+    return e;
   }
   const dim3 grid((W + 63) / 64, N * H, Cout / (16 * nt));
   switch (nt) {
@@ -585,9 +610,10 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
     case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
     default: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu); break;
   }
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess || !pool.p) return e;
-  return avgpool2(out, pool, N, H, W, Cout, s);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && pool.p) e = avgpool2(out, pool, N, H, W, Cout, s);
+  if (e == hipSuccess && fuse.y) e = residual_loss(fuse.res, out, nullptr, fuse.y, nullptr, nullptr, N * H * W, 0.0f, s);
+  return e;
 }
 
 hipError_t conv3x3_wgrad(View x, View dy, float* dw, float* db, int N, int H, int W, int Cin, int Cout,

@@ -22,14 +22,24 @@ struct View {
 //   out[n,y,x,k] = act(bias[k] + sum_{r,s,c} in[n, y+r-1, x+s-1, c] * w[r][s][k][c])
 // w: bf16 [3][3][Cout][Cin]; bias: fp32 [Cout] or null; act: ReLU if relu.
 // Cin and Cout multiples of 16, channel strides/offsets multiples of 4.
-// With Cup > 0 the first Cup input channels are the nearest-neighbour 2x
-// upsampling of `up` (an N x H/2 x W/2 tensor), read in place, and `in`
-// holds channels Cup..Cin-1 (Cup a multiple of 32, H and W even, 16-byte
-// aligned channel runs in both views).  With pool.p set, the 2x2 average
-// of the output (as avgpool2 computes it) is also written to `pool`.
+struct ConvFuse {
+  // Cup > 0: the first Cup input channels are the nearest-neighbour 2x
+  // upsampling of `up` (an N x H/2 x W/2 tensor), read in place, and `in`
+  // holds channels Cup..Cin-1 (Cup a multiple of 32, H and W even, 16-byte
+  // aligned channel runs in both views).
+  View up{nullptr, 0, 0};
+  int Cup = 0;
+  // pool.p set: the 2x2 average of the output (as avgpool2 computes it) is
+  // also written to `pool`.
+  View pool{nullptr, 0, 0};
+  // y set: instead of `out`, y[p][c] = res[p][c] + out[p][c] for c < 4
+  // (fp32, 4 channels per pixel, the output rounded to bf16 first, as
+  // residual_loss computes it).
+  View res{nullptr, 0, 0};
+  float* y = nullptr;
+};
 hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, int H, int W, int Cin, int Cout,
-                   int relu, hipStream_t s, View up = View{nullptr, 0, 0}, int Cup = 0,
-                   View pool = View{nullptr, 0, 0});
+                   int relu, hipStream_t s, const ConvFuse& fuse = ConvFuse{});
 // Weight gradient: dw[r][s][k][c] += sum_p dy[p,k] * x[p + (r-1, s-1), c] and
 // db[k] += sum_p dy[p,k] (fp32, accumulated with atomics; zero them first).
 hipError_t conv3x3_wgrad(View x, View dy, float* dw, float* db, int N, int H, int W, int Cin, int Cout,
