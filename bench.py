@@ -26,7 +26,7 @@ KERNELS = {1: "k_fast<quad>", 2: "k_fast<wave-aggregated>", 3: "k_fast<naive>", 
 # HBM bytes per draw launch from rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (scale factors from a 512 MiB copy), profiles/k_fast_cfg2_hbm_counters.md.
 # Keyed by (config, hist mode, waves per block); other keys report null.
-MEASURED_TRAFFIC = {("cfg2", 0, 8): 5.41e9, ("cfg2", 5, 8): 5.41e9, ("cfg2", 4, 8): 8.76e9, ("cfg2", 1, 4): 9.06e9}
+MEASURED_TRAFFIC = {("cfg2", 0, 8): 5.32e9, ("cfg2", 5, 8): 5.32e9, ("cfg2", 4, 8): 8.76e9, ("cfg2", 1, 4): 9.06e9}
 
 
 def parse():
