@@ -143,7 +143,10 @@ __global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
   const int R = g.R;
   const int SW = DE_TW + 2 * R, SH = DE_TH + 2 * R;
   float4* s_col = reinterpret_cast<float4*>(smem);
-  int* s_r = reinterpret_cast<int*>(smem + sizeof(float4) * SW * SH);  // source radius, -1: empty
+  // s_m[d * SH + row]: bit c set when the source in tile column c of that row
+  // has a radius of at least d (SW <= 64 columns for R <= 16)
+  unsigned long long* s_m = reinterpret_cast<unsigned long long*>(smem + sizeof(float4) * SW * SH);
+  int* s_r = reinterpret_cast<int*>(s_m + (R + 1) * SH);  // source radius, -1: empty
   float* s_wt = reinterpret_cast<float*>(s_r + SW * SH);
   __shared__ int s_rmax;
   const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * DE_TH;
@@ -157,6 +160,7 @@ __global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
     m.nrm = (r > 0) ? rf_div_rn(0.63661977236f, (float)(r * r)) : 1.0f;
     for (int q = tid; q < S * S; q += DE_TW * DE_TH) s_wt[de_wt_off(r) + q] = de_weight(m, q / S - r, q % S - r);
   }
+  for (int q = tid; q < (R + 1) * SH; q += DE_TW * DE_TH) s_m[q] = 0ull;
   __syncthreads();
   int rloc = 0;
   for (int k = tid; k < SW * SH; k += DE_TW * DE_TH) {
@@ -166,6 +170,7 @@ __global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
     s_col[k] = col;
     s_r[k] = (int)m.r;
     rloc = max(rloc, (int)m.r);
+    for (int d = 0; d <= (int)m.r; ++d) atomicOr(&s_m[d * SH + ly], 1ull << lx);
   }
   // A source contributes only within its own radius (de_weight is 0 beyond
   // it), so offsets beyond the largest radius staged for this tile add
@@ -185,15 +190,26 @@ __global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
   const int tx = tx0 + lx, ty = ty0 + ly;
   if (tx < (int)g.W && ty < (int)g.H) {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    // Only sources whose radius reaches the row offset are visited (the row's
+    // mask for |i|), in the order of the full (i, j) sweep: j ascending is
+    // tile column descending, so the highest set bit comes first.  The terms
+    // left out are the zero ones the sweep skipped, so the sum is the same.
+    const int c_lo = lx + R - RM;  // tile columns c_lo .. c_lo + 2 RM, j = lx + R - c
+    const unsigned long long wmask = (2ull << (2 * RM)) - 1ull;
     for (int i = -RM; i <= RM; ++i) {
       // source row cy = ty - i -> LDS row ly + R - i
-      const int row = (ly + R - i) * SW;
+      const int rowi = ly + R - i;
+      const int row = rowi * SW;
       const int ai = i < 0 ? -i : i;
-      for (int j = -RM; j <= RM; ++j) {
-        const int k = row + lx + R - j;
+      unsigned long long wm = (s_m[ai * SH + rowi] >> c_lo) & wmask;
+      while (wm) {
+        const int b = 63 - __clzll((long long)wm);
+        wm &= ~(1ull << b);
+        const int k = row + c_lo + b;
+        const int j = RM - b;
         const int r = s_r[k];
         const int aj = j < 0 ? -j : j;
-        if (r >= ai && r >= aj) {
+        if (r >= aj) {
           const float wgt = s_wt[de_wt_off(r) + (i + r) * (2 * r + 1) + (j + r)];
           if (wgt != 0.0f) {
             const float4 c = s_col[k];
@@ -295,7 +311,8 @@ hipError_t launch_animate(Block* blocks, uint32_t T, float tss_width, const floa
 hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s) {
   const int SW = DE_TW + 2 * a.R, SH = DE_TH + 2 * a.R;
   const size_t n_wt = (size_t)(4 * (a.R + 1) * (a.R + 1) * (a.R + 1) - (a.R + 1)) / 3;  // de_wt_off(R + 1)
-  const size_t sm = (size_t)SW * SH * (sizeof(float4) + sizeof(int)) + n_wt * sizeof(float);
+  const size_t sm = (size_t)SW * SH * (sizeof(float4) + sizeof(int)) + (size_t)(a.R + 1) * SH * 8 +
+                    n_wt * sizeof(float);
   if (sm <= 65536) {  // within the default dynamic-LDS limit of one launch (R <= 14)
     dim3 grid((a.W + DE_TW - 1) / DE_TW, (a.H + DE_TH - 1) / DE_TH);
     hipLaunchKernelGGL(k_de_tiled, grid, dim3(DE_TW * DE_TH), sm, s, a);
