@@ -485,10 +485,12 @@ static bool sort_eligible(const rf_flame* f, uint32_t W, uint32_t H) {
 }
 
 // Capacities of the next sort draw from the last one's distribution (20 %
-// headroom), or 3x the mean where there is no history; uploads offsets and
-// capacities and grows the record buffers.  meta layout: boff, bcap, toff, tcap.
-static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, rfk::SortArgs& a,
-                        uint32_t& max_bcap, std::vector<uint32_t>& meta) {
+// headroom).  Without one, from the samples the histogram already holds (the
+// first draw in AUTO mode is an LDS draw into it), else 3x the mean.  Uploads
+// offsets and capacities and grows the record buffers.  meta layout: boff,
+// bcap, toff, tcap.
+static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n_bins, uint64_t S,
+                        rfk::SortArgs& a, uint32_t& max_bcap, std::vector<uint32_t>& meta) {
   const uint32_t nb = (uint32_t)((n_bins + (1u << rfk::SORT_COARSE_SHIFT) - 1) >> rfk::SORT_COARSE_SHIFT);
   const uint32_t nt = nb * rfk::SORT_TILES_PER_BUCKET;
   const uint32_t nt_used = (uint32_t)((n_bins + (1u << rfk::SORT_TILE_SHIFT) - 1) >> rfk::SORT_TILE_SHIFT);
@@ -523,6 +525,23 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, uint32_t n_bins, uint64_t S, r
     f->sort_bfrac.assign(nb, 3.0 / nb);
     f->sort_tfrac.assign(nt, 0.0);
     for (uint32_t t = 0; t < nt_used; ++t) f->sort_tfrac[t] = 3.0 / nt_used;
+    if (hist) {
+      if (ctx->sort_mass.bytes < (size_t)nt_used * 4) HIPCHK(ctx, ctx->sort_mass.alloc((size_t)nt * 4));
+      std::vector<float> mass(nt_used);
+      HIPCHK(ctx, rfk::launch_sort_tile_mass(hist, n_bins, nt_used, (float*)ctx->sort_mass.p, ctx->stream));
+      HIPCHK(ctx, hipMemcpyAsync(mass.data(), ctx->sort_mass.p, (size_t)nt_used * 4, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      double total = 0.0;
+      for (float m : mass) total += m;
+      if (total > 0.0) {
+        f->sort_bfrac.assign(nb, 0.0);
+        for (uint32_t t = 0; t < nt_used; ++t) {
+          f->sort_tfrac[t] = mass[t] / total;
+          f->sort_bfrac[t / rfk::SORT_TILES_PER_BUCKET] += mass[t] / total;
+        }
+      }
+    }
   }
   meta.assign(2 * (size_t)nb + 2 * (size_t)nt, 0u);
   uint32_t* boff = meta.data();
@@ -627,6 +646,22 @@ static constexpr double kAutoSortMaxHits = 0.90;
 
 static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, const uint32_t* rows_in,
                    uint64_t* drawn) {
+  // AUTO mode's first draw of a flame measures the LDS table's re-hits on one
+  // pass and runs the other passes in the mode that measurement picks (a
+  // whole 7680x4320 draw through the LDS table costs five sort draws).
+  if (ctx && f && h && ctx->sp.engine == RF_ENGINE_FAST && ctx->sp.hist_mode == RF_HIST_AUTO &&
+      f->auto_mode == RF_HIST_LDS && !f->auto_probed && num_iter > 1 && !rows_in) {
+    f->auto_probed = true;
+    uint64_t d1 = 0, d2 = 0;
+    int rc = do_draw(ctx, f, h, 1, nullptr, &d1);
+    if (rc) return rc;
+    const float ms1 = ctx->last_draw_ms;
+    rc = do_draw(ctx, f, h, num_iter - 1, nullptr, &d2);
+    if (rc) return rc;
+    ctx->last_draw_ms += ms1;
+    if (drawn) *drawn = d1 + d2;
+    return RF_OK;
+  }
   if (!ctx || !f || !h || f->ctx != ctx || h->ctx != ctx) return set_err(ctx, RF_ERR_ARG, "bad context, flame or histogram");
   if (!f->have_particles || f->generation != ctx->generation || f->needs_update)
     return set_err(ctx, RF_ERR_STATE, "flame needs a warmup before draw_to_bins");
@@ -679,7 +714,7 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     uint32_t max_bcap = 0;
     const uint64_t S = P * (uint64_t)num_iter;
     if (hm_run == RF_HIST_SORT && num_iter) {
-      const int rc2 = sort_prepare(ctx, f, h->w * h->h, S, a.sort, max_bcap, meta);
+      const int rc2 = sort_prepare(ctx, f, (const float4*)h->ptr, h->w * h->h, S, a.sort, max_bcap, meta);
       if (rc2) return rc2;
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));

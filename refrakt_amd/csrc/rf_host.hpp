@@ -81,7 +81,7 @@ struct rf_ctx {
   int perm_waves = 0;
   rfh::DevBuf counter;         // u64 [drawn, samples added in the LDS bin cache]
   // RF_HIST_SORT buffers: bucket and tile records, cursors, offsets, capacities
-  rfh::DevBuf sort_cbuf, sort_tbuf, sort_bcur, sort_tcur;
+  rfh::DevBuf sort_cbuf, sort_tbuf, sort_bcur, sort_tcur, sort_mass;
   rfh::DevBuf probe;           // atomic probe scratch
   rf_hist* render_hist = nullptr;
   rfh::DevBuf render_img;
@@ -108,6 +108,7 @@ struct rf_flame {
   // kAutoLdsMinHits of its samples in LDS.
   int auto_mode = RF_HIST_LDS;
   double last_lds_hit_fraction = -1.0;
+  bool auto_probed = false;  // AUTO's first draw has measured the LDS table on one pass
   // RF_HIST_SORT: fraction of the samples that went to each bucket and tile
   // in the last draw (sizes the next draw's capacities), and the fraction of
   // binned samples that overflowed them.

@@ -247,7 +247,33 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(SortArgs a, float4*
   }
 }
 
+// One workgroup per tile: the sum of the count channel over its bins, the
+// estimate of where the next draw's samples go when a flame has no sort
+// draw to learn from yet.
+__global__ __launch_bounds__(256) void k_sort_tile_mass(const float4* __restrict__ hist, uint32_t n_bins,
+                                                       float* __restrict__ mass) {
+  __shared__ float s_w[256 / WAVE];
+  const uint32_t base = blockIdx.x * (uint32_t)TILE_BINS;
+  float sum = 0.0f;
+  for (uint32_t i = threadIdx.x; i < (uint32_t)TILE_BINS; i += 256u)
+    if (base + i < n_bins) sum += hist[base + i].w;
+  sum = wave_sum(sum);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_w[threadIdx.x / WAVE] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.0f;
+    for (int k = 0; k < 256 / WAVE; ++k) t += s_w[k];
+    mass[blockIdx.x] = t;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_sort_tile_mass(const float4* bins, uint32_t n_bins, uint32_t n_tiles, float* mass, hipStream_t s) {
+  if (n_tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_tile_mass, dim3(n_tiles), dim3(256), 0, s, bins, n_bins, mass);
+  return hipGetLastError();
+}
 
 hipError_t launch_sort_reset(const SortArgs& a, hipStream_t s) {
   const uint32_t n = a.n_tiles > a.n_buckets ? a.n_tiles : a.n_buckets;

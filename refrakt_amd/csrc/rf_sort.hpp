@@ -51,6 +51,8 @@ struct SortArgs {
 hipError_t launch_sort_reset(const SortArgs& a, hipStream_t s);
 hipError_t launch_sort_rebucket(const SortArgs& a, float* bins, const float4* palette, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, float* bins, uint32_t n_bins, const float4* palette, hipStream_t s);
+// mass[t] = sum of the count channel over tile t of a histogram (t < n_tiles).
+hipError_t launch_sort_tile_mass(const float4* bins, uint32_t n_bins, uint32_t n_tiles, float* mass, hipStream_t s);
 #endif
 
 }  // namespace rfk
