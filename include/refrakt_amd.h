@@ -67,8 +67,8 @@ extern "C" {
 
 /* Histogram accumulation strategies of the fast engine (the compat engine
  * uses QUAD for AUTO and LDS). */
-#define RF_HIST_AUTO 0     /* LDS first; after one draw SORT if under 90 % of the samples re-hit the LDS
-                              tables (QUAD if SORT is not allowed and under 2 %) */
+#define RF_HIST_AUTO 0     /* LDS for one pass of the first draw; then SORT if under 90 % of its samples
+                              re-hit the LDS tables (QUAD if SORT is not allowed and under 2 %) */
 #define RF_HIST_QUAD 1     /* 4 lanes per RGBA bin: one 16-byte atomic request per sample */
 #define RF_HIST_WAVEAGG 2  /* merge same-bin hits inside a wave before the atomics */
 #define RF_HIST_NAIVE 3    /* 4 separate float atomics per sample (baseline) */
