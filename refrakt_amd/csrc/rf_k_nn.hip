@@ -82,30 +82,34 @@ __global__ __launch_bounds__(256) void k_conv3x3(View in, const bf16* __restrict
 __device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
-// LDS-tiled implicit GEMM.  A workgroup of 4 waves covers CT_R = 4 image rows
-// x CT_W = 64 pixels (one row per wave, four 16-pixel A fragments) and 16*NT
-// output channels.  Per KC-channel step of Cin (KC = 32 with gfx950's
+// LDS-tiled implicit GEMM.  A workgroup of 4 waves covers 4 RW image rows
+// x CT_W = 64 pixels (RW rows per wave, four 16-pixel A fragments per row) and
+// 16*NT output channels.  Per KC-channel step of Cin (KC = 32 with gfx950's
 // v_mfma_f32_16x16x32_bf16 when Cin allows, else 16 with the 16x16x16 form)
-// it stages the (CT_R + 2) x (CT_W + 2) input patch (zero halo) and the
+// it stages the (4 RW + 2) x (CT_W + 2) input patch (zero halo) and the
 // step's 3x3 x 16*NT x KC weights in LDS, KC + 8 channels per pixel / weight
 // row (the pad spreads 16 consecutive rows over distinct banks and keeps
 // 16-byte stores aligned); every A fragment then feeds NT MFMAs and every B
-// fragment four.  The next step's operands are loaded into registers while
-// the MFMAs of the current one run.  Needs 16-byte aligned channel runs (Cin,
+// fragment four per output row it reaches.  With RW = 2 a wave's four input
+// rows serve six (row, tap row) pairs, so each A fragment read from LDS feeds
+// up to 2 NT MFMAs: the narrow layers, whose NT is small, read a third fewer
+// A fragments and half the B fragments per output.  The next step's operands
+// are loaded into registers while the MFMAs of the current one run.  Needs 16-byte aligned channel runs (Cin,
 // channel strides and offsets multiples of 8).
 // Operand layouts (wave64, i = lane % 16, kq = lane / 16): 16x16x16 lane
 // holds A[i][4 kq + e], B[4 kq + e][i] (e < 4); 16x16x32 lane holds
 // A[i][8 kq + e], B[8 kq + e][i] (e < 8); both give D[4 kq + e][i] (e < 4).
-constexpr int CT_W = 64, CT_R = 4;
-constexpr int CT_IN_ROWS = (CT_R + 2) * (CT_W + 2);  // staged input pixels
+constexpr int CT_W = 64;
+// staged input pixels of a tile of 4 RW rows
+__host__ __device__ constexpr int ct_in_pix(int RW) { return (4 * RW + 2) * (CT_W + 2); }
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
-template <int NT, int KC>
+template <int NT, int KC, int RW>
 struct ConvStage {
   static constexpr int PS = KC + 8;             // LDS row stride (bf16)
   static constexpr int PQ = KC / 8;             // 16-byte pieces per pixel / weight row
-  static constexpr int IN_Q = CT_IN_ROWS * PQ;
+  static constexpr int IN_Q = ct_in_pix(RW) * PQ;
   static constexpr int W_Q = 9 * 16 * NT * PQ;
   static constexpr int IN_PER = (IN_Q + 255) / 256;
   static constexpr int W_PER = (W_Q + 255) / 256;
@@ -116,11 +120,11 @@ struct ConvStage {
 // Channels [0, Cup) come from the half-resolution `up` (nearest-neighbour
 // upsampling read in place), the rest from `in`; a KC step never straddles
 // the two (Cup is a multiple of 32).
-template <int NT, int KC>
-__device__ __forceinline__ void conv_load(ConvStage<NT, KC>& st, const View& in, const View& up, int Cup,
+template <int NT, int KC, int RW>
+__device__ __forceinline__ void conv_load(ConvStage<NT, KC, RW>& st, const View& in, const View& up, int Cup,
                                           const bf16* __restrict__ w, int n, int y0, int x0, int co0, int c0, int H,
                                           int W, int Cin, int Cout) {
-  using S = ConvStage<NT, KC>;
+  using S = ConvStage<NT, KC, RW>;
   const int tid = threadIdx.x;
   const bool from_up = c0 < Cup;  // workgroup-uniform
 #pragma unroll
@@ -152,79 +156,40 @@ __device__ __forceinline__ void conv_load(ConvStage<NT, KC>& st, const View& in,
   }
 }
 
+// One lane's operand fragment: 4 (16x16x16) or 8 (16x16x32) bf16 values.
 template <int KC>
-__device__ __forceinline__ float4_t conv_mfma(const bf16* ap, const bf16* bp, float4_t acc);
+struct ConvFrag;
 template <>
-__device__ __forceinline__ float4_t conv_mfma<16>(const bf16* ap, const bf16* bp, float4_t acc) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const short4_t*>(ap),
-                                                   *reinterpret_cast<const short4_t*>(bp), acc, 0, 0, 0);
-}
-template <>
-__device__ __forceinline__ float4_t conv_mfma<32>(const bf16* ap, const bf16* bp, float4_t acc) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8_t*>(ap),
-                                                 *reinterpret_cast<const bf16x8_t*>(bp), acc, 0, 0, 0);
-}
-
-template <int NT, int KC>
-__global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __restrict__ w,
-                                                     const float* __restrict__ bias, View out, int H, int W,
-                                                     int Cin, int Cout, int relu, int vec_out, ConvFuse fz) {
-  const View& up = fz.up;
-  const int Cup = fz.Cup;
-  const View& pool = fz.pool;
-  using S = ConvStage<NT, KC>;
-  constexpr int PS = S::PS, KE = KC / 4;  // fragment elements per lane
-  constexpr int OC = 16 * NT, OS = OC + 8;  // output tile: channels per pixel, LDS row stride
-  static_assert(4 * CT_W * OS <= CT_IN_ROWS * PS + 9 * 16 * NT * PS, "output tile fits the staging buffer");
-  __shared__ __attribute__((aligned(16))) bf16 s_buf[CT_IN_ROWS * PS + 9 * 16 * NT * PS];
-  bf16* const s_in = s_buf;
-  bf16* const s_w = s_buf + CT_IN_ROWS * PS;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nrb = (H + CT_R - 1) / CT_R;
-  const int n = blockIdx.y / nrb, y0 = (blockIdx.y % nrb) * CT_R;
-  const int x0 = blockIdx.x * CT_W;
-  const int co0 = blockIdx.z * 16 * NT;
-  const int i = lane & 15, kq = lane >> 4;
-  float4_t acc[4][NT];
-#pragma unroll
-  for (int f = 0; f < 4; ++f)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[f][t] = float4_t{0.f, 0.f, 0.f, 0.f};
-  S st;
-  conv_load<NT, KC>(st, in, up, Cup, w, n, y0, x0, co0, 0, H, W, Cin, Cout);
-  for (int c0 = 0; c0 < Cin; c0 += KC) {
-    __syncthreads();  // the previous step's fragment reads are done
-#pragma unroll
-    for (int k = 0; k < S::IN_PER; ++k) {
-      const int q = tid + 256 * k;
-      if (q < S::IN_Q) *reinterpret_cast<uint4*>(s_in + (q / S::PQ) * PS + 8 * (q % S::PQ)) = st.vin[k];
-    }
-#pragma unroll
-    for (int k = 0; k < S::W_PER; ++k) {
-      const int q = tid + 256 * k;
-      if (q < S::W_Q) *reinterpret_cast<uint4*>(s_w + (q / S::PQ) * PS + 8 * (q % S::PQ)) = st.vw[k];
-    }
-    __syncthreads();
-    if (c0 + KC < Cin) conv_load<NT, KC>(st, in, up, Cup, w, n, y0, x0, co0, c0 + KC, H, W, Cin, Cout);
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        const bf16* bp = s_w + ((r * 3 + s) * 16 * NT + i) * PS + KE * kq;
-        const bf16* ap = s_in + ((wave + r) * (CT_W + 2) + i + s) * PS + KE * kq;
-#pragma unroll
-        for (int f = 0; f < 4; ++f)
-#pragma unroll
-          for (int t = 0; t < NT; ++t) acc[f][t] = conv_mfma<KC>(ap + 16 * f * PS, bp + 16 * t * PS, acc[f][t]);
-      }
-    }
+struct ConvFrag<16> {
+  typedef short4_t T;
+  static __device__ __forceinline__ float4_t mfma(T a, T b, float4_t acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
   }
-  const int y = y0 + wave;
-  if (vec_out) {  // workgroup-uniform
-    // Through LDS, so that each pixel's 16*NT output channels leave as 16-byte
-    // stores instead of one 2-byte store per lane and value.
-    __syncthreads();  // the last step's fragment reads are done: the staging buffer is free
-    bf16* const s_o = s_buf + wave * CT_W * OS;
+};
+template <>
+struct ConvFrag<32> {
+  typedef bf16x8_t T;
+  static __device__ __forceinline__ float4_t mfma(T a, T b, float4_t acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+  }
+};
+
+// Output of one tile: bias, ReLU and bf16 in LDS (tile row RW * wave + rr is
+// image row y0 + RW * wave + rr), then 16-byte stores per pixel, the fused
+// 2x2 averages or the fused fp32 residual.  The output tile reuses the
+// staging buffer: the caller's next step waits on a barrier before staging.
+template <int NT, int RW>
+__device__ __forceinline__ void conv_store_lds(const float4_t (&acc)[RW][4][NT], bf16* s_buf,
+                                               const float* __restrict__ bias, const View& out, const ConvFuse& fz,
+                                               int relu, int n, int y0, int x0, int co0, int H, int W) {
+  constexpr int OC = 16 * NT, OS = OC + 8, TR = 4 * RW;
+  constexpr int PQ = OC / 8;  // 16-byte pieces per pixel
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i = lane & 15, kq = lane >> 4;
+  __syncthreads();  // the last step's fragment reads are done: the staging buffer is free
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    bf16* const s_o = s_buf + (RW * wave + rr) * CT_W * OS;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const float bv = bias ? bias[co0 + 16 * t + i] : 0.0f;
@@ -232,36 +197,45 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __rest
       for (int f = 0; f < 4; ++f)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          float v = acc[f][t][e] + bv;
+          float v = acc[rr][f][t][e] + bv;
           if (relu) v = fmaxf(v, 0.0f);
           s_o[(16 * f + 4 * kq + e) * OS + 16 * t + i] = f2bf(v);
         }
     }
-    __syncthreads();
-    constexpr int PQ = OC / 8;  // 16-byte pieces per pixel
-    if (fz.y) {  // workgroup-uniform: the residual output replaces `out`
-      const int px = x0 + lane;
+  }
+  __syncthreads();
+  if (fz.y) {  // workgroup-uniform: the residual output replaces `out`
+    const int px = x0 + lane;
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) {
+      const int y = y0 + RW * wave + rr;
       if (y < H && px < W) {
         const size_t p = (size_t)(n * H + y) * W + px;
         const uint2 xv = *reinterpret_cast<const uint2*>(fz.res.p + p * fz.res.cs + fz.res.co);
-        const uint2 ov = *reinterpret_cast<const uint2*>(s_o + lane * OS);
+        const uint2 ov = *reinterpret_cast<const uint2*>(s_buf + ((RW * wave + rr) * CT_W + lane) * OS);
         *reinterpret_cast<float4*>(fz.y + p * 4) =
             make_float4(bf_lo(xv.x) + bf_lo(ov.x), bf_hi(xv.x) + bf_hi(ov.x), bf_lo(xv.y) + bf_lo(ov.y),
                         bf_hi(xv.y) + bf_hi(ov.y));
       }
-      return;
     }
-    if (y < H) {
+  } else {
 #pragma unroll
-      for (int q = lane; q < CT_W * PQ; q += 64) {
-        const int px = q / PQ, piece = q % PQ;
-        if (x0 + px < W)
-          *reinterpret_cast<uint4*>(out.p + ((size_t)(n * H + y) * W + x0 + px) * out.cs + out.co + co0 + 8 * piece) =
-              *reinterpret_cast<const uint4*>(s_o + px * OS + 8 * piece);
+    for (int rr = 0; rr < RW; ++rr) {
+      const int y = y0 + RW * wave + rr;
+      const bf16* const s_o = s_buf + (RW * wave + rr) * CT_W * OS;
+      if (y < H) {
+#pragma unroll
+        for (int q = lane; q < CT_W * PQ; q += 64) {
+          const int px = q / PQ, piece = q % PQ;
+          if (x0 + px < W)
+            *reinterpret_cast<uint4*>(out.p + ((size_t)(n * H + y) * W + x0 + px) * out.cs + out.co + co0 +
+                                      8 * piece) = *reinterpret_cast<const uint4*>(s_o + px * OS + 8 * piece);
+        }
       }
     }
-    if (pool.p) {  // workgroup-uniform: the tile's 2x2 averages, summed in k_avgpool2_v8's order
-      constexpr int PW = CT_W / 2, NPQ = (CT_R / 2) * PW * PQ;
+    if (fz.pool.p) {  // workgroup-uniform: the tile's 2x2 averages, summed in k_avgpool2_v8's order
+      const View& pool = fz.pool;
+      constexpr int PW = CT_W / 2, NPQ = (TR / 2) * PW * PQ;
       for (int q = tid; q < NPQ; q += 256) {
         const int piece = q % PQ, pp = (q / PQ) % PW, pr = q / (PQ * PW);
         const int yo = (y0 >> 1) + pr, xo = (x0 >> 1) + pp;
@@ -286,22 +260,133 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(View in, const bf16* __rest
         }
       }
     }
-    return;
   }
-  if (y >= H) return;
+}
+
+// One workgroup per tile (x block, row block of 4 RW rows of image n, 16 NT
+// output channel block): workgroup t has channel block t % nbz, x block
+// (t / nbz) % nbx and row block t / (nbz nbx), so the workgroups that read
+// the same input patch for different output channels run next to each other.
+// (A persistent variant, each workgroup walking tiles b, b + G, ... with the
+// next tile's first step prefetched during the last step of the current one,
+// measured slower: 16.8 ms against 14.0 ms per 7680x4320 inference, its
+// registers no longer fitting three waves per SIMD without spills.)
+template <int NT, int KC, int RW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RW == 2 ? 3 : 1))) void k_conv3x3_lds(
+    View in, const bf16* __restrict__ w, const float* __restrict__ bias, View out, int H, int W, int Cin, int Cout,
+    int relu, int vec_out, ConvFuse fz, int nbx, int nbz, int n_tiles) {
+  const View& up = fz.up;
+  const int Cup = fz.Cup;
+  using S = ConvStage<NT, KC, RW>;
+  using F = ConvFrag<KC>;
+  constexpr int PS = S::PS, KE = KC / 4;  // fragment elements per lane
+  constexpr int OC = 16 * NT, OS = OC + 8;  // output tile: channels per pixel, LDS row stride
+  constexpr int TR = 4 * RW;                // tile rows
+  constexpr int IN_PIX = ct_in_pix(RW);
+  static_assert(TR * CT_W * OS <= IN_PIX * PS + 9 * 16 * NT * PS, "output tile fits the staging buffer");
+  __shared__ __attribute__((aligned(16))) bf16 s_buf[IN_PIX * PS + 9 * 16 * NT * PS];
+  bf16* const s_in = s_buf;
+  bf16* const s_w = s_buf + IN_PIX * PS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nrb = (H + TR - 1) / TR;
+  const int i = lane & 15, kq = lane >> 4;
+  const int tile = blockIdx.x;
+  if (tile >= n_tiles) return;  // workgroup-uniform
+  const int bz = tile % nbz, rest = tile / nbz;
+  const int bx = rest % nbx, by = rest / nbx;
+  const int n = by / nrb, y0 = (by % nrb) * TR, x0 = bx * CT_W, co0 = bz * 16 * NT;
+  float4_t acc[RW][4][NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int co = co0 + 16 * t + i;
-    const float bv = bias ? bias[co] : 0.0f;
+  for (int rr = 0; rr < RW; ++rr)
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
+    for (int f = 0; f < 4; ++f)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int px = x0 + 16 * f + 4 * kq + e;
-        if (px < W) {
-          float v = acc[f][t][e] + bv;
-          if (relu) v = fmaxf(v, 0.0f);
-          out.p[((size_t)(n * H + y) * W + px) * out.cs + out.co + co] = f2bf(v);
+      for (int t = 0; t < NT; ++t) acc[rr][f][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+  S st;
+  conv_load<NT, KC, RW>(st, in, up, Cup, w, n, y0, x0, co0, 0, H, W, Cin, Cout);
+  for (int c0 = 0; c0 < Cin; c0 += KC) {
+    __syncthreads();  // the previous step's fragment reads are done
+#pragma unroll
+    for (int k = 0; k < S::IN_PER; ++k) {
+      const int q = tid + 256 * k;
+      if (q < S::IN_Q) *reinterpret_cast<uint4*>(s_in + (q / S::PQ) * PS + 8 * (q % S::PQ)) = st.vin[k];
+    }
+#pragma unroll
+    for (int k = 0; k < S::W_PER; ++k) {
+      const int q = tid + 256 * k;
+      if (q < S::W_Q) *reinterpret_cast<uint4*>(s_w + (q / S::PQ) * PS + 8 * (q % S::PQ)) = st.vw[k];
+    }
+    __syncthreads();
+    if (c0 + KC < Cin) conv_load<NT, KC, RW>(st, in, up, Cup, w, n, y0, x0, co0, c0 + KC, H, W, Cin, Cout);
+    if constexpr (RW == 1) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const bf16* bp = s_w + ((r * 3 + s) * 16 * NT + i) * PS + KE * kq;
+          const bf16* ap = s_in + ((wave + r) * (CT_W + 2) + i + s) * PS + KE * kq;
+#pragma unroll
+          for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+              acc[0][f][t] = F::mfma(*reinterpret_cast<const typename F::T*>(ap + 16 * f * PS),
+                                     *reinterpret_cast<const typename F::T*>(bp + 16 * t * PS), acc[0][f][t]);
+        }
+    } else {
+      // Staged input row j of the wave's RW + 2 rows is tap row r = j - rr
+      // of its output row rr: per tap column s, the 3 NT B fragments are
+      // read once and every A fragment once, for all the rows it reaches.
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        typename F::T b[3][NT];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            b[r][t] = *reinterpret_cast<const typename F::T*>(s_w + ((r * 3 + s) * 16 * NT + 16 * t + i) * PS +
+                                                              KE * kq);
+#pragma unroll
+        for (int j = 0; j < RW + 2; ++j) {
+          const bf16* ap = s_in + ((RW * wave + j) * (CT_W + 2) + i + s) * PS + KE * kq;
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            const typename F::T a = *reinterpret_cast<const typename F::T*>(ap + 16 * f * PS);
+#pragma unroll
+            for (int rr = 0; rr < RW; ++rr) {
+              const int r = j - rr;
+              if (r >= 0 && r < 3) {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[rr][f][t] = F::mfma(a, b[r][t], acc[rr][f][t]);
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  if (vec_out) {  // workgroup-uniform
+    conv_store_lds<NT, RW>(acc, s_buf, bias, out, fz, relu, n, y0, x0, co0, H, W);
+  } else {
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) {
+      const int y = y0 + RW * wave + rr;
+      if (y < H) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const int co = co0 + 16 * t + i;
+          const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int px = x0 + 16 * f + 4 * kq + e;
+              if (px < W) {
+                float v = acc[rr][f][t][e] + bv;
+                if (relu) v = fmaxf(v, 0.0f);
+                out.p[((size_t)(n * H + y) * W + px) * out.cs + out.co + co] = f2bf(v);
+              }
+            }
+          }
         }
       }
     }
@@ -569,7 +654,18 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
       return hipErrorInvalidValue;
   }
   if (lds) {
-    const dim3 grid((W + CT_W - 1) / CT_W, N * ((H + CT_R - 1) / CT_R), Cout / (16 * nt));
+    // K = 32 steps pay off where each staged step feeds 64 output channels;
+    // the narrow full-resolution layers keep the smaller K = 16 tiles, whose
+    // LDS and registers fit four workgroups per CU (measured: 32-channel
+    // layers 1.15 ms with K = 16 against 1.53 ms with K = 32 at 7680x4320),
+    // and give each wave two rows (RW = 2), whose shared input rows halve the
+    // B and cut the A fragment reads per output by a third.
+    const bool k32 = Cin % 32 == 0 && nt == 4;
+    const int rw = nt <= 2 ? 2 : 1;
+    const int nbx = (W + CT_W - 1) / CT_W, nbz = Cout / (16 * nt);
+    const long long tiles = (long long)nbx * N * ((H + 4 * rw - 1) / (4 * rw)) * nbz;
+    if (tiles >= (1ll << 24)) return hipErrorInvalidValue;  // a 1-D grid of 256-thread workgroups
+    const int n_tiles = (int)tiles;
     const int vec_out = out.cs % 8 == 0 && out.co % 8 == 0 && ((uintptr_t)out.p & 15u) == 0;
     // the fused pooling needs the LDS epilogue; otherwise pool afterwards
     const bool fuse_pool = pool.p && vec_out && H % 2 == 0 && W % 2 == 0 && pool.cs % 8 == 0 &&
@@ -581,20 +677,16 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
     ConvFuse fz = fuse;
     if (!fuse_pool) fz.pool = View{nullptr, 0, 0};
     if (!fuse_res) { fz.res = View{nullptr, 0, 0}; fz.y = nullptr; }
-#define RF_CONV_LDS(NT_, KC_)                                                                                  \
-  hipLaunchKernelGGL((k_conv3x3_lds<NT_, KC_>), grid, dim3(256), 0, s, in, w, bias, out, H, W, Cin, Cout, relu, \
-                     vec_out, fz)
-    // K = 32 steps pay off where each staged step feeds 64 output channels;
-    // the narrow full-resolution layers keep the smaller K = 16 tiles, whose
-    // LDS and registers fit four workgroups per CU (measured: 32-channel
-    // layers 1.15 ms with K = 16 against 1.53 ms with K = 32 at 7680x4320).
-    if (Cin % 32 == 0 && nt == 4) {
-      RF_CONV_LDS(4, 32);
+#define RF_CONV_LDS(NT_, KC_, RW_)                                                                            \
+  hipLaunchKernelGGL((k_conv3x3_lds<NT_, KC_, RW_>), dim3((unsigned)n_tiles), dim3(256), 0, s, in, w, bias, out, H, \
+                     W, Cin, Cout, relu, vec_out, fz, nbx, nbz, n_tiles)
+    if (k32) {
+      RF_CONV_LDS(4, 32, 1);
     } else {
       switch (nt) {
-        case 4: RF_CONV_LDS(4, 16); break;
-        case 2: RF_CONV_LDS(2, 16); break;
-        default: RF_CONV_LDS(1, 16); break;
+        case 4: RF_CONV_LDS(4, 16, 1); break;
+        case 2: RF_CONV_LDS(2, 16, 2); break;
+        default: RF_CONV_LDS(1, 16, 2); break;
       }
     }
 #undef RF_CONV_LDS
