@@ -207,8 +207,8 @@ def bench_batch(rf, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
-                                   "design note, see DESIGN.md)"}))
+        print(json.dumps({"error": "this renderer runs on one GPU; --gpus must be 1 (a job would shard by "
+                                   "particles, each GPU summing a private histogram, added once at the end)"}))
         sys.exit(2)
     import torch
     if args.copy_calibration:

@@ -86,7 +86,7 @@ bool param_lookup(const char* name, int* var, int* k) {
   return true;
 }
 
-// Random permutation of [0, n) (Fisher-Yates), the build's stand-in for the
+// Random permutation of [0, n) (Fisher-Yates), this library's stand-in for the
 // reference's std::shuffle rows (src/shuffle_buffers.cpp:9-21).
 void shuffle_row(uint32_t* row, uint32_t n, uint64_t seed) {
   uint64_t s = seed;
