@@ -623,6 +623,18 @@ __global__ void k_log_density(const float* __restrict__ hist, bf16* __restrict__
   out[i] = c < 4 ? f2bf(log10f(1.0f + hist[(size_t)p * 4 + c] * scale)) : (bf16)0;
 }
 
+// The same values, one pixel per thread: one 16-byte load of the bin and two
+// 16-byte stores of its 16 channels (16-byte aligned buffers).
+__global__ void k_log_density_v(const float4* __restrict__ hist, uint4* __restrict__ out, int npix, float scale) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  const float4 h = hist[p];
+  const uint32_t lo = (uint32_t)f2bf(log10f(1.0f + h.x * scale)) | ((uint32_t)f2bf(log10f(1.0f + h.y * scale)) << 16);
+  const uint32_t hi = (uint32_t)f2bf(log10f(1.0f + h.z * scale)) | ((uint32_t)f2bf(log10f(1.0f + h.w * scale)) << 16);
+  out[2 * (size_t)p] = make_uint4(lo, hi, 0u, 0u);
+  out[2 * (size_t)p + 1] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 __global__ void k_bf16_to_f32_4(const bf16* __restrict__ in, int cs, float* __restrict__ out, int npix) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npix * 4) return;
@@ -773,7 +785,12 @@ hipError_t to_bf16(const float* in, bf16* out, int n, hipStream_t s) {
 }
 
 hipError_t log_density_bf16(const float* hist, bf16* out, int npix, float scale, hipStream_t s) {
-  hipLaunchKernelGGL(k_log_density, grid1((size_t)npix * 16), dim3(256), 0, s, hist, out, npix, scale);
+  if (npix < 0 || (size_t)npix * 16 > (size_t)INT32_MAX) return hipErrorInvalidValue;
+  if ((((uintptr_t)hist | (uintptr_t)out) & 15u) == 0)
+    hipLaunchKernelGGL(k_log_density_v, grid1((size_t)npix), dim3(256), 0, s, reinterpret_cast<const float4*>(hist),
+                       reinterpret_cast<uint4*>(out), npix, scale);
+  else
+    hipLaunchKernelGGL(k_log_density, grid1((size_t)npix * 16), dim3(256), 0, s, hist, out, npix, scale);
   return hipGetLastError();
 }
 

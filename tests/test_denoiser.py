@@ -122,6 +122,32 @@ def test_conv3x3_gradients_match_torch(rf, gpu_available, Cin, Cout, W):
     assert (err <= 2.0 ** -8 * ref_dx.abs() + 1e-4).all(), float(err.max())
 
 
+@pytest.mark.parametrize("npix", [1, 257, 4096 + 3])
+def test_log_density_input_matches_formula(rf, gpu_available, npix):
+    """bf16(log10(1 + v scale)) in channels 0..3, zeros in 4..15; the
+    one-pixel-per-thread kernel (16-byte aligned buffers) and the per-value
+    kernel (an output 2 bytes off alignment) give the same bits."""
+    import torch
+    ctx = rf.Context(1 << 12, 1, 16)
+    g = torch.Generator().manual_seed(npix)
+    h = torch.rand(npix, 4, generator=g) * torch.tensor([40.0, 3.0, 0.5, 900.0])
+    h[::7] = 0.0
+    scale = 0.37
+    hd = h.cuda()
+    out_a = torch.full((npix * 16,), -1.0, dtype=torch.bfloat16, device="cuda")
+    out_b = torch.full((npix * 16 + 1,), -1.0, dtype=torch.bfloat16, device="cuda")
+    ctx.log_density_input(hd.data_ptr(), out_a.data_ptr(), npix, scale)
+    ctx.log_density_input(hd.data_ptr(), out_b[1:].data_ptr(), npix, scale)
+    a = out_a.cpu().view(npix, 16)
+    b = out_b[1:].cpu().view(npix, 16)
+    assert torch.equal(a.view(torch.int16), b.view(torch.int16))
+    assert (a[:, 4:] == 0).all()
+    ref = torch.log10(1.0 + h.double() * np.float32(scale))
+    err = (a[:, :4].double() - ref).abs()
+    assert (err <= 2.0 ** -8 * ref.abs() + 1e-6).all(), float(err.max())
+    assert float(out_b[0].float()) == -1.0  # nothing written before the view
+
+
 def test_adam_matches_formula(rf, gpu_available):
     import torch
     ctx = rf.Context(1 << 12, 1, 16)
