@@ -387,6 +387,46 @@ def test_sort_path_matches_quad(rf, gpu_available, cfg, size):
         assert o2 < 0.01
 
 
+@pytest.mark.parametrize("size", [(1, 1), (3, 5), (67, 45)])
+def test_tiny_and_ragged_histograms_every_mode(rf, gpu_available, size):
+    """A zero-pass draw returns 0 and adds nothing; on 1x1, 3x5 and 67x45
+    histograms every histogram mode (quad, wave aggregation, naive, LDS table,
+    sort) adds the samples the quad path adds: equal alpha counts, `drawn`
+    equal to their sum, colours within the modes' rounding.  AUTO's first draw
+    of a flame runs its first pass as a launch of its own (the LDS probe), and
+    every launch seeds its own streams, so its samples differ from a single
+    launch's: there `drawn` equals the sum of its alpha counts."""
+    from refrakt_amd import configs
+    wl = configs.cfg1()
+    W, H = size
+    fl = wl.flame
+    fl.size = [W, H]
+    fl.scale = fl.scale * W / wl.width
+    out = {}
+    for mode in (1, 2, 3, 4, 5, 0):
+        ctx = _ctx(rf, 1 << 14, 1, rf.RF_ENGINE_FAST, hist_mode=mode, seed=5, waves=8)
+        fh = ctx.flame(fl)
+        hist = ctx.histogram(W, H)
+        fh.warmup(3)
+        assert fh.draw_to_bins(hist, 0) == 0
+        assert not hist.download().any()
+        d = fh.draw_to_bins(hist, 3)
+        out[mode] = (d, hist.download())
+        fh.close()
+        ctx.close()
+    dq, hq = out[1]
+    assert dq == int(hq[..., 3].astype(np.float64).sum()) > 0
+    n = hq[..., 3:4].astype(np.float64)
+    tol = 1e-3 + n * 2.0 ** -16 + np.abs(hq[..., :3]) * (2e-3 + n * 2.0 ** -23)
+    for mode, (d, h) in out.items():
+        assert d == int(h[..., 3].astype(np.float64).sum()), mode
+        if mode == 0:
+            continue
+        assert d == dq, mode
+        assert np.array_equal(h[..., 3], hq[..., 3]), mode
+        assert (np.abs(h[..., :3].astype(np.float64) - hq[..., :3]) <= tol).all(), mode
+
+
 def test_full_size_cfg2_conservation(rf, gpu_available):
     """cfg2 at its real size (7680x4320, 2^24 particles, 2^32 binned samples)."""
     from refrakt_amd import configs
