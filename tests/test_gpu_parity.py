@@ -427,6 +427,44 @@ def test_tiny_and_ragged_histograms_every_mode(rf, gpu_available, size):
         assert (np.abs(h[..., :3].astype(np.float64) - hq[..., :3]) <= tol).all(), mode
 
 
+@pytest.mark.parametrize("H", [8192, 8193])
+def test_sort_path_at_its_size_limit(rf, gpu_available, H):
+    """16384 x 8192 = 2^27 bins is the sort path's largest histogram (256
+    buckets of 2^19 bins): sort-mode draws give the quad path's alpha counts
+    exactly (first draw on uniform capacities, so hot buckets overflow into
+    the atomic path; second on learned ones).  One row more and sort-mode
+    draws fall back to quad atomics, with the same result."""
+    import torch
+    from refrakt_amd import configs
+    wl = configs.cfg2()
+    W = 16384
+    fl = wl.flame
+    fl.size = [W, H]
+    fl.scale = fl.scale * W / wl.width
+    res = {}
+    for mode in (1, 5):
+        ctx = _ctx(rf, 1 << 18, 1, rf.RF_ENGINE_FAST, hist_mode=mode, seed=3, waves=8)
+        buf = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        hist = ctx.histogram(W, H, buf.data_ptr())
+        fh = ctx.flame(fl)
+        fh.warmup(5)
+        d = fh.draw_to_bins(hist, 4)
+        d += fh.draw_to_bins(hist, 4)
+        res[mode] = (d, buf)
+        if mode == 5:  # -1: no sort draw ran
+            assert (fh.sort_overflow() >= 0.0) == (W * H <= 1 << 27)
+        hist.close()
+        fh.close()
+        ctx.close()
+    (dq, bq), (ds, bs) = res[1], res[5]
+    assert dq == ds > 0
+    assert torch.equal(bq[..., 3], bs[..., 3])
+    assert int(bq[..., 3].double().sum()) == dq
+    n = bq[..., 3:4].double()
+    tol = 1e-3 + n * 2.0 ** -16 + bq[..., :3].double().abs() * (2e-3 + n * 2.0 ** -23)
+    assert bool(((bs[..., :3].double() - bq[..., :3].double()).abs() <= tol).all())
+
+
 def test_full_size_cfg2_conservation(rf, gpu_available):
     """cfg2 at its real size (7680x4320, 2^24 particles, 2^32 binned samples)."""
     from refrakt_amd import configs
