@@ -43,6 +43,22 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+hipError_t HostBuf::alloc(size_t n) {
+  if (p && bytes >= n) return hipSuccess;
+  release();
+  if (n == 0) return hipSuccess;
+  hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+  if (e != hipSuccess) { p = nullptr; bytes = 0; return e; }
+  bytes = n;
+  return hipSuccess;
+}
+
+void HostBuf::release() {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
 const rf_var_info* var_info(int id) {
   if (id < 0 || id >= RF_GEN_NUM_VARIATIONS) return nullptr;
   return &k_var_table[id];
@@ -611,11 +627,10 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
 }
 
 // After a sort draw: record where the samples went and how many overflowed.
+// bc, tc: the draw's bucket and tile cursors, already copied to the host.
 static int sort_finish(rf_ctx* ctx, rf_flame* f, const rfk::SortArgs& a, uint64_t S, uint64_t drawn,
-                       const std::vector<uint32_t>& meta) {
-  std::vector<uint32_t> bc(a.n_buckets), tc(a.n_tiles);
-  HIPCHK(ctx, hipMemcpy(bc.data(), a.bcur, bc.size() * 4, hipMemcpyDeviceToHost));
-  HIPCHK(ctx, hipMemcpy(tc.data(), a.tcur, tc.size() * 4, hipMemcpyDeviceToHost));
+                       const std::vector<uint32_t>& meta, const uint32_t* bc, const uint32_t* tc) {
+  (void)ctx;
   const uint32_t* bcap = meta.data() + a.n_buckets;
   const uint32_t* tcap = meta.data() + 2 * (size_t)a.n_buckets + a.n_tiles;
   uint64_t ovf = 0;
@@ -733,16 +748,27 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     sort_S = S;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-  uint64_t cnt[2] = {0, 0};
-  uint64_t slots[rfk::COUNTER_SLOTS * rfk::COUNTER_STRIDE];
-  HIPCHK(ctx, hipMemcpyAsync(slots, ctx->counter.p, counter_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  // counter slots, then the sort cursors, copied back with one synchronisation
+  const size_t cur_words = sort_used ? (size_t)sort_args.n_buckets + sort_args.n_tiles : 0;
+  HIPCHK(ctx, ctx->readback.alloc(counter_bytes + cur_words * 4));
+  const uint64_t* slots = static_cast<const uint64_t*>(ctx->readback.p);
+  uint32_t* cur_host = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->readback.p) + counter_bytes);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->readback.p, ctx->counter.p, counter_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (sort_used) {
+    HIPCHK(ctx, hipMemcpyAsync(cur_host, sort_args.bcur, (size_t)sort_args.n_buckets * 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(cur_host + sort_args.n_buckets, sort_args.tcur, (size_t)sort_args.n_tiles * 4,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  uint64_t cnt[2] = {0, 0};
   for (int k = 0; k < rfk::COUNTER_SLOTS; ++k) {
     cnt[0] += slots[k * rfk::COUNTER_STRIDE];
     cnt[1] += slots[k * rfk::COUNTER_STRIDE + 1];
   }
   if (sort_used) {
-    const int rc2 = sort_finish(ctx, f, sort_args, sort_S, cnt[0], sort_meta);
+    const int rc2 = sort_finish(ctx, f, sort_args, sort_S, cnt[0], sort_meta, cur_host,
+                                cur_host + sort_args.n_buckets);
     if (rc2) return rc2;
   }
   if (ctx->sp.engine == RF_ENGINE_FAST && hm == RF_HIST_LDS && cnt[0]) {

@@ -39,6 +39,19 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// Pinned host buffer (hipHostMalloc): device-to-host copies into it run
+// asynchronously, so a draw's readbacks share one stream synchronisation.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t alloc(size_t n);
+  void release();
+  ~HostBuf() { release(); }
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+};
+
 // Builds the device Block of a flame (weights normalised over the non-final
 // xforms, terms in name order, pre_xform terms first).  Returns false and
 // sets err on a limit violation.
@@ -82,6 +95,8 @@ struct rf_ctx {
   rfh::DevBuf counter;         // u64 [drawn, samples added in the LDS bin cache]
   // RF_HIST_SORT buffers: bucket and tile records, cursors, offsets, capacities
   rfh::DevBuf sort_cbuf, sort_tbuf, sort_bcur, sort_tcur, sort_mass;
+  // a draw's readbacks: counter slots, then (sort draws) bucket and tile cursors
+  rfh::HostBuf readback;
   rfh::DevBuf probe;           // atomic probe scratch
   rf_hist* render_hist = nullptr;
   rfh::DevBuf render_img;
