@@ -628,9 +628,8 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
 
 // After a sort draw: record where the samples went and how many overflowed.
 // bc, tc: the draw's bucket and tile cursors, already copied to the host.
-static int sort_finish(rf_ctx* ctx, rf_flame* f, const rfk::SortArgs& a, uint64_t S, uint64_t drawn,
-                       const std::vector<uint32_t>& meta, const uint32_t* bc, const uint32_t* tc) {
-  (void)ctx;
+static void sort_finish(rf_flame* f, const rfk::SortArgs& a, uint64_t S, uint64_t drawn,
+                        const std::vector<uint32_t>& meta, const uint32_t* bc, const uint32_t* tc) {
   const uint32_t* bcap = meta.data() + a.n_buckets;
   const uint32_t* tcap = meta.data() + 2 * (size_t)a.n_buckets + a.n_tiles;
   uint64_t ovf = 0;
@@ -645,7 +644,6 @@ static int sort_finish(rf_ctx* ctx, rf_flame* f, const rfk::SortArgs& a, uint64_
     f->sort_tfrac[t] = tc[t] * inv;
   }
   f->last_sort_overflow = drawn ? (double)ovf / (double)drawn : 0.0;
-  return RF_OK;
 }
 
 // RF_HIST_AUTO keeps the LDS bin cache while at least this fraction of the
@@ -766,11 +764,7 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     cnt[0] += slots[k * rfk::COUNTER_STRIDE];
     cnt[1] += slots[k * rfk::COUNTER_STRIDE + 1];
   }
-  if (sort_used) {
-    const int rc2 = sort_finish(ctx, f, sort_args, sort_S, cnt[0], sort_meta, cur_host,
-                                cur_host + sort_args.n_buckets);
-    if (rc2) return rc2;
-  }
+  if (sort_used) sort_finish(f, sort_args, sort_S, cnt[0], sort_meta, cur_host, cur_host + sort_args.n_buckets);
   if (ctx->sp.engine == RF_ENGINE_FAST && hm == RF_HIST_LDS && cnt[0]) {
     f->last_lds_hit_fraction = (double)cnt[1] / (double)cnt[0];
     if (ctx->sp.hist_mode == RF_HIST_AUTO && f->last_lds_hit_fraction < kAutoSortMaxHits) {
