@@ -126,8 +126,10 @@ typedef struct rf_flame_desc {
 
 /* set_sim_parameters (src/flame.cpp:105-158) plus this library's choices. */
 typedef struct rf_sim_params {
-  uint64_t total_particles;   /* P; a multiple of 256 * temporal_samples */
-  uint32_t temporal_samples;  /* T; a multiple of 32 (animate dispatch, shaders/templates/animate.tpl.glsl) */
+  uint64_t total_particles;   /* P; a multiple of 256 * temporal_samples, below 2^32 */
+  uint32_t temporal_samples;  /* T >= 1.  The reference animates T/32 groups of 32 samples
+                                 (shaders/templates/animate.tpl.glsl); here any T works and
+                                 T = 1 is the centre sample (no rotation). */
   uint32_t shuffle_count;     /* shuffle rows (compat engine); 0 -> 1024 */
   uint64_t seed;              /* seeds RNG states, shuffle rows and the row schedule */
   int32_t device;             /* HIP device ordinal */
@@ -165,10 +167,10 @@ typedef struct rf_render_params {
 typedef struct rf_stats {
   uint64_t drawn;             /* binned samples (the reference's atomic counter) */
   uint64_t iterations;        /* particle iterations, binned or not */
-  double ms_warmup;
-  double ms_draw;
-  double ms_density;
-  double ms_tonemap;
+  double ms_warmup;           /* host clock: rf_warmup */
+  double ms_draw;             /* host clock: the draw_calls rf_draw_to_bins calls */
+  double ms_density;          /* device time of the density-estimation kernel */
+  double ms_tonemap;          /* device time of the tonemap kernel */
 } rf_stats;
 
 /* ------------------------------------------------------------- context */
@@ -244,8 +246,11 @@ int rf_tonemap(rf_ctx* ctx, const void* in_device, void* out_device,
 /* Fused density estimation + tonemap of a histogram into out_device. */
 int rf_density_tonemap(rf_ctx* ctx, rf_hist* bins, const rf_de_params* de,
                        const rf_tonemap_params* tm, void* out_device);
-/* The whole path: warmup, draw_calls x draw_to_bins, DE, tonemap.  hist_out
- * and image_out are host arrays of W*H*4 floats (either may be NULL). */
+/* The whole path (src/main.cpp:403-535): zero the context's histogram,
+ * warmup, draw_calls x draw_to_bins, DE, tonemap (two kernels, as the
+ * reference's two passes; the image equals rf_density_tonemap of the same
+ * histogram).  hist_out and image_out are host arrays of W*H*4 floats (either
+ * may be NULL). */
 int rf_render(rf_ctx* ctx, rf_flame* flame, const rf_render_params* params,
               float* hist_out, float* image_out, rf_stats* stats);
 

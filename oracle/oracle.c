@@ -475,20 +475,23 @@ int orc_select(const orc_flame* f, float u) {      /* xform_select.tpl.glsl:1-14
 
 void orc_animate(const orc_flame* src, uint32_t i, uint32_t T, float tss, orc_flame* dst) {
   *dst = *src;                                     /* animate.tpl.glsl:18-96 */
-  int half = (int)(T / 2u);
-  if (half == 0) half = 1;
-  int pos = (int)i - half;
-  float dt = (float)pos / (float)half * tss;
-  const float rps = 0.31415926535f;
+  /* half_width = T/2 (animate.tpl.glsl:19, T/32 groups of 32 lanes,
+   * flame.cpp:248); the reference leaves T < 32 undefined, here T = 1 is the
+   * centre sample (dt = 0).  sin/cos are evaluated in double and rounded, the
+   * division is correctly rounded (the GPU kernel does the same). */
+  const int half = (int)(T / 2u);
+  const int pos = (int)i - half;                   /* animate.tpl.glsl:20 */
+  const float dt = half ? (float)pos / (float)half * tss : 0.0f;   /* :21 */
+  const float rps = 0.31415926535f;                /* :16 */
   for (int k = 0; k <= src->n_xforms; ++k) {
     const orc_xform* s;
     orc_xform* d;
     if (k < src->n_xforms) { s = &src->xf[k]; d = &dst->xf[k]; }
     else if (src->has_final) { s = &src->fin; d = &dst->fin; }
     else break;
-    float th = rps * dt * s->rotation_frequency;
-    float sn = sinf(th), cs = cosf(th);
-    const float* a = s->aff;
+    float th = rps * dt * s->rotation_frequency;   /* :43-44 */
+    float sn = (float)sin((double)th), cs = (float)cos((double)th);
+    const float* a = s->aff;                       /* :45-48 */
     float t0 = a[0] * cs, t1 = a[2] * sn, t2 = a[1] * cs, t3 = a[3] * sn;
     float t4 = a[2] * cs, t5 = a[0] * sn, t6 = a[3] * cs, t7 = a[1] * sn;
     d->aff[0] = t0 + t1; d->aff[1] = t2 + t3; d->aff[2] = t4 - t5; d->aff[3] = t6 - t7;

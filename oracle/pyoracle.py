@@ -208,6 +208,14 @@ def var_eval(name, x, y, w=1.0, params=None, affine=(1, 0, 0, 1, 0, 0), rands=No
     return ox.value, oy.value
 
 
+def animate(flame_c, i, T, tss):
+    """Temporal sample i of T (animate.tpl.glsl:18-96): a copy of flame_c with
+    rotated affines."""
+    out = OFlame()
+    lib().orc_animate(C.byref(flame_c), i, T, tss, C.byref(out))
+    return out
+
+
 def compat_pass(flame_c, T, ppt, pos_in, pos_out, shuf, rs, in_row, out_row, random_read, random_write, first_run,
                 do_draw, W=1, H=1, ss=None, bins=None, hits=None, flames_per_sample=None):
     L = lib()

@@ -138,6 +138,21 @@ class Frame:
         return self.spp * self.width * self.height
 
 
+DEGREES_PER_SECOND = 18.0
+
+
+def rotate_affine(a, deg):
+    """flame::rotate_affine (src/flame.hpp:97-109) in float32: the linear part
+    [a c; b d] times the rotation [cos -sin; sin cos] on the right,
+    a' = a cos + c sin, b' = b cos + d sin, c' = c cos - a sin, d' = d cos - b sin."""
+    f = np.float32
+    rad = f(0.01745329251) * f(deg)
+    sn, cs = f(math.sin(rad)), f(math.cos(rad))
+    a = [f(v) for v in a]
+    out = [a[0] * cs + a[2] * sn, a[1] * cs + a[3] * sn, a[2] * cs - a[0] * sn, a[3] * cs - a[1] * sn, a[4], a[5]]
+    return [float(v) for v in out]
+
+
 CFG4_SIZES = [(512, 512), (1024, 1024), (1920, 1080), (2048, 2048)]
 
 
@@ -161,13 +176,11 @@ def cfg4(seed: int = 4, frames: int = 64) -> List[Frame]:
         spp = int(rng.integers(64, 513))
         xs = []
         for x in fam[:n]:
-            th = 2 * math.pi * t * x["rate"]
-            c, s_ = math.cos(th), math.sin(th)
-            a, b, cc, d, e, f = x["aff"]
-            # rotate the linear part: [a c; b d] <- R(th) [a c; b d]
-            ra, rb = c * a - s_ * b, s_ * a + c * b
-            rc, rd = c * cc - s_ * d, s_ * cc + c * d
-            xs.append(Xform(affine=[ra, rb, rc, rd, e, f], variations=dict(x["vars"]), weight=x["weight"],
+            # the reference's animation clock (src/main.cpp:383-395): 18 degrees
+            # per second x rotation_frequency; t in [0, 1] spans 20 s, one turn
+            # at rate 1
+            aff = rotate_affine(x["aff"], DEGREES_PER_SECOND * 20.0 * t * x["rate"])
+            xs.append(Xform(affine=aff, variations=dict(x["vars"]), weight=x["weight"],
                             color=x["color"], color_speed=0.5, opacity=1.0))
         fl = Flame(xforms=xs, palette=pal, size=[W, H], center=[0.0, 0.0], scale=min(W, H) / 4.0, rotate=0.0,
                    estimator_min=0, estimator_radius=9, estimator_curve=0.4, gamma=2.2, vibrancy=1.0,

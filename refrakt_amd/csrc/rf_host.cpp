@@ -280,7 +280,8 @@ static int validate_sim(rf_ctx* ctx, uint64_t P, uint32_t T) {
   if (T == 0) return set_err(ctx, RF_ERR_ARG, "temporal_samples must be positive");
   if (P == 0 || P % (256ull * T) != 0)
     return set_err(ctx, RF_ERR_ARG, "total_particles must be a positive multiple of 256 * temporal_samples");
-  if (P / T > 0xffffffffull) return set_err(ctx, RF_ERR_LIMIT, "particles per temporal sample exceed 2^32");
+  // particle and batch indices are 32-bit in both engines
+  if (P > 0xffffffffull) return set_err(ctx, RF_ERR_LIMIT, "total_particles must be below 2^32");
   return RF_OK;
 }
 
@@ -375,7 +376,11 @@ static int flame_upload(rf_flame* f, float tss_width) {
   HIPCHK(ctx, hipMemcpyAsync(f->palette.p, f->desc.palette, 256 * 16, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   f->needs_update = false;
+  // Every warmup starts AUTO's choice afresh: the edited flame's first draw
+  // probes the LDS table again and reports a new re-hit fraction.
   f->auto_mode = fixed_point_ok(f->desc) ? RF_HIST_LDS : RF_HIST_QUAD;
+  f->auto_probed = false;
+  f->last_lds_hit_fraction = -1.0;
   return RF_OK;
 }
 
@@ -1172,10 +1177,28 @@ int rf_render(rf_ctx* ctx, rf_flame* f, const rf_render_params* rp, float* hist_
     s.drawn += d;
   }
   auto t2 = clk::now();
+  // The reference's two passes (src/main.cpp:490-515 density estimation into
+  // a texture, src/main.cpp:522-535 tonemap of that texture), timed apart on
+  // the device.  Without density estimation the radius-0 splat copies each
+  // bin into the render-target layout.  rf_density_tonemap fuses both.
   rf_de_params none{0, 0, 0.0f};
   HIPCHK(ctx, ctx->render_img.alloc((size_t)rp->width * rp->height * 16));
-  if ((rc = rf_density_tonemap(ctx, h, rp->density_estimation ? &rp->de : &none, &rp->tm, ctx->render_img.p))) return rc;
-  auto t3 = clk::now();
+  if (!ctx->ev[0]) { HIPCHK(ctx, hipEventCreate(&ctx->ev[0])); HIPCHK(ctx, hipEventCreate(&ctx->ev[1])); }
+  rfk::DEArgs da = de_args(h, rp->density_estimation ? &rp->de : &none);
+  da.de_out = (float4*)ctx->render_img.p;
+  da.do_tonemap = 0;
+  HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  HIPCHK(ctx, rfk::launch_de_tonemap(da, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPCHK(ctx, rfk::launch_tonemap((const float4*)ctx->render_img.p, (float4*)ctx->render_img.p,
+                                  rp->width * rp->height, rp->tm.gamma, rp->tm.brightness, rp->tm.vibrancy,
+                                  rp->tm.scale_constant, ctx->stream));
+  if (!ctx->ev[2]) { HIPCHK(ctx, hipEventCreate(&ctx->ev[2])); HIPCHK(ctx, hipEventCreate(&ctx->ev[3])); }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  HIPCHK(ctx, hipEventSynchronize(ctx->ev[2]));
+  float de_ms = 0.0f, tm_ms = 0.0f;
+  HIPCHK(ctx, hipEventElapsedTime(&de_ms, ctx->ev[0], ctx->ev[1]));
+  HIPCHK(ctx, hipEventElapsedTime(&tm_ms, ctx->ev[1], ctx->ev[2]));
   if (hist_out) if ((rc = rf_hist_download(h, hist_out))) return rc;
   if (image_out) {
     HIPCHK(ctx, hipMemcpy(image_out, ctx->render_img.p, (size_t)rp->width * rp->height * 16, hipMemcpyDeviceToHost));
@@ -1183,8 +1206,8 @@ int rf_render(rf_ctx* ctx, rf_flame* f, const rf_render_params* rp, float* hist_
   s.iterations = ctx->sp.total_particles * ((uint64_t)rp->warmup_passes + 1 + (uint64_t)rp->draw_passes * rp->draw_calls);
   s.ms_warmup = ms(t0, t1);
   s.ms_draw = ms(t1, t2);
-  s.ms_density = ms(t2, t3);
-  s.ms_tonemap = 0.0;
+  s.ms_density = de_ms;
+  s.ms_tonemap = tm_ms;
   if (st) *st = s;
   return RF_OK;
 }

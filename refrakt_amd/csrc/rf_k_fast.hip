@@ -26,20 +26,25 @@ static hipError_t launch_fast_k(const FastArgs& a0, uint64_t n, hipStream_t s) {
   a.n_batches = (uint32_t)(n / (64u * W));
   uint32_t grid = a.n_batches;
   if (HM == 4) {
-    // one workgroup per resident slot (cached per instantiation), so that each
-    // LDS table spans many batches before it is flushed
-    static int resident = 0;
-    if (!resident) {
-      int dev = 0, cus = 0, per_cu = 0;
-      hipError_t e = hipGetDevice(&dev);
-      if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // one workgroup per resident slot (cached per instantiation and device),
+    // so that each LDS table spans many batches before it is flushed
+    constexpr int kMaxDev = 64;
+    static int resident[kMaxDev] = {0};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    int res = dev < kMaxDev ? resident[dev] : 0;
+    if (!res) {
+      int cus = 0, per_cu = 0;
+      e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
       if (e == hipSuccess)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_fast<W, HM, LOGK>),
                                                          64 * W, 0);
       if (e != hipSuccess) return e;
-      resident = cus * (per_cu > 0 ? per_cu : 1);
+      res = cus * (per_cu > 0 ? per_cu : 1);
+      if (dev < kMaxDev) resident[dev] = res;
     }
-    grid = std::min<uint32_t>(grid, (uint32_t)resident);
+    grid = std::min<uint32_t>(grid, (uint32_t)res);
   }
   hipLaunchKernelGGL((k_fast<W, HM, LOGK>), dim3(grid), dim3(64 * W), 0, s, a, a.blocks);
   return hipGetLastError();

@@ -17,6 +17,11 @@ using namespace detail;
 // dt = (i - T/2) / float(T/2) * tss:
 //   a' = a cos + c sin, b' = b cos + d sin, c' = c cos - a sin, d' = d cos - b sin.
 // rot[k] is xform k's rotation_frequency (rot[n_xforms] for the final xform).
+// The reference dispatches T/32 groups of 32 lanes, so it defines T >= 32
+// only; at T = 1 (half = 0) the single block is the centre sample, dt = 0.
+// The division is correctly rounded and sin/cos are evaluated in double and
+// rounded to float, so the blocks are a deterministic function of the inputs
+// that a CPU computes to the same bits.
 __global__ __launch_bounds__(256) void k_animate(Block* blocks, uint32_t T, float tss,
                                                  const float* __restrict__ rot) {
   const uint32_t i = blockIdx.x;
@@ -32,10 +37,10 @@ __global__ __launch_bounds__(256) void k_animate(Block* blocks, uint32_t T, floa
   const int slot = k < nx ? k : FINAL;
   const int half = (int)(T / 2u);
   const int pos = (int)i - half;
-  const float dt = (float)pos / (float)half * tss;
+  const float dt = half ? rf_div_rn((float)pos, (float)half) * tss : 0.0f;
   const float rps = 0.31415926535f;
   const float th = rps * dt * rot[k];
-  const float sn = sinf(th), cs = cosf(th);
+  const float sn = (float)sin((double)th), cs = (float)cos((double)th);
   const float* a = src->xf[slot].aff;
   float* b = dst->xf[slot].aff;
   const float t0 = a[0] * cs, t1 = a[2] * sn;

@@ -128,6 +128,87 @@ def test_fast_bit_exact_vs_oracle_replica(rf, orc, gpu_available, hist_mode):
         np.testing.assert_allclose(g_hist[..., :3], bins[..., :3], rtol=1e-5, atol=1e-5)
 
 
+# ------------------------------------------------- non-finite particle policy
+def _nan_xform(rf, kind):
+    """An xform whose output is never finite: a NaN variation weight, or an
+    affine that sends every point beyond 1e10 (the badval bound,
+    shaders/include/math.glsl)."""
+    if kind == "nan":
+        return rf.Xform(affine=[0.5, 0.0, 0.0, 0.5, 0.0, 0.0], variations={"linear": float("nan")}, weight=0.35,
+                        color=0.9, color_speed=0.5, opacity=1.0)
+    return rf.Xform(affine=[1e30, 0.0, 0.0, 1e30, 0.0, 0.0], variations={"linear": 1.0}, weight=0.35, color=0.9,
+                    color_speed=0.5, opacity=1.0)
+
+
+@pytest.mark.parametrize("kind", ["nan", "huge"])
+@pytest.mark.parametrize("hist_mode", [1, 5])
+def test_fast_nonfinite_particles_restart(rf, orc, gpu_available, kind, hist_mode):
+    """The fast engine restarts a particle whose position is NaN or beyond
+    1e10 at a uniform point of [-1, 1]^2 and does not draw it that iteration
+    (the reference leaves it NaN forever: the reset at shaders/flame.glsl:70 is
+    commented out).  Checked bit-exactly against the oracle's replica, which
+    applies the same policy, and by its invariants: every particle finite
+    afterwards, alpha sum = drawn."""
+    P, T, W, H, seed = 1 << 13, 1, 192, 160, 77
+    fl = _linear_flame(rf, 23, W=W, H=H)
+    fl.xforms.append(_nan_xform(rf, kind))
+    print(f"nonfinite restart: kind={kind} hist_mode={hist_mode}", flush=True)
+    ctx = _ctx(rf, P, T, rf.RF_ENGINE_FAST, hist_mode=hist_mode, seed=seed)
+    fh = ctx.flame(fl)
+    hist = ctx.histogram(W, H)
+    fh.warmup(2)
+    drawn = fh.draw_to_bins(hist, 6)
+    g_hist = hist.download()
+    g_pos = fh.particles()
+    perms = ctx.fast_perms(4)
+    assert np.isfinite(g_pos).all() and (np.abs(g_pos[:, :2]) <= 1e10).all()
+    assert int(g_hist[..., 3].sum(dtype=np.float64)) == drawn > 0
+
+    fc = orc.flame_to_c(fl)
+    ss = orc.screen_affine(fl, W, H)
+    pos = np.zeros((P, 4), np.float32)
+    orc.fast_run(fc, P // T, P, 4, perms, seed, 0, 3, True, False, W, H, ss, pos)
+    hits = np.zeros((H, W), np.uint32)
+    o_drawn = orc.fast_run(fc, P // T, P, 4, perms, seed, 1, 6, False, True, W, H, ss, pos, None, hits)
+    assert drawn == o_drawn
+    assert np.array_equal(g_pos.view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(g_hist[..., 3].astype(np.uint32), hits)
+
+
+def test_fast_only_nonfinite_xform_draws_nothing(rf, gpu_available):
+    """A flame whose one xform never yields a finite point: every iteration
+    restarts every particle, nothing is drawn, and the particles end inside
+    [-1, 1]^2."""
+    W, H = 128, 128
+    fl = _linear_flame(rf, 25, n=1, final=False, post=False, W=W, H=H)
+    fl.xforms[0] = _nan_xform(rf, "nan")
+    ctx = _ctx(rf, 1 << 13, 1, rf.RF_ENGINE_FAST, hist_mode=1, seed=3)
+    fh = ctx.flame(fl)
+    hist = ctx.histogram(W, H)
+    fh.warmup(3)
+    assert fh.draw_to_bins(hist, 4) == 0
+    assert not hist.download().any()
+    p = fh.particles()
+    assert np.isfinite(p).all() and (np.abs(p[:, :2]) <= 1.0).all()
+
+
+def test_auto_probe_runs_again_after_rewarmup(rf, gpu_available):
+    """RF_HIST_AUTO measures the LDS table on the first pass of a flame's
+    first draw after every warmup (an edited flame may spread differently)."""
+    W, H = 640, 480
+    fl = _linear_flame(rf, 27, W=W, H=H)
+    ctx = _ctx(rf, 1 << 16, 1, rf.RF_ENGINE_FAST, hist_mode=0, seed=5, waves=8)
+    fh = ctx.flame(fl)
+    hist = ctx.histogram(W, H)
+    for k in range(2):
+        fh.warmup(3)
+        mode, frac = fh.hist_mode()
+        assert mode == 4 and frac < 0, (k, mode, frac)      # LDS table, nothing measured yet
+        assert fh.draw_to_bins(hist, 4) > 0
+        mode, frac = fh.hist_mode()
+        assert frac >= 0 and mode == 5, (k, mode, frac)     # probed, moved to the sort path
+
+
 # --------------------------------------------------------------- variations
 DISCONTINUOUS = {"rings", "fan", "fan2", "rings2", "julian", "juliascope", "pie", "ngon", "rectangles", "boarders",
                  "cell", "cpow", "secant2", "twintrian", "bipolar", "oscope", "julia", "loonie", "bent", "bent2",
