@@ -51,6 +51,7 @@ extern "C" {
 #define RF_MAX_XFORMS 32          /* non-final xforms per flame */
 #define RF_MAX_VARS_PER_XFORM 16  /* variation terms per xform */
 #define RF_MAX_VAR_PARAMS 8       /* parameters per variation */
+#define RF_NOALLOC_SCRATCH_BYTES 8192  /* device scratch of rf_draw_to_bins_noalloc, per GPU */
 #define RF_PALETTE_SIZE 256
 
 /* ---------------------------------------------------------------- engines */
@@ -237,6 +238,22 @@ int rf_warmup_rows(rf_ctx* ctx, rf_flame* flame, uint32_t num_passes,
                    float tss_width, const uint32_t* rows);
 int rf_draw_to_bins_rows(rf_ctx* ctx, rf_flame* flame, rf_hist* bins,
                          uint32_t num_iter, const uint32_t* rows, uint64_t* drawn);
+/* Allocation-free rf_draw_to_bins.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Calls no
+ * hipMalloc, hipFree or other allocating routine: the draw counters live in
+ * a static device array of RF_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends, its address taken per device by rf_create) and are cleared
+ * by a kernel on every call.  The caller passes a warmed-up flame and a
+ * histogram as for rf_draw_to_bins; the fast engine only (RF_ERR_ARG
+ * otherwise), in a direct histogram mode: RF_HIST_SORT, and AUTO's sort
+ * choice, run the LDS bin table instead (no record buffers), and AUTO never
+ * probes.  Uses a flame program only if one is already compiled (else the
+ * precompiled interpreter, same results).  Results equal rf_draw_to_bins in
+ * the same mode; rf_last_draw_ms reads -1 after it.  Calls that overlap on
+ * one GPU (any contexts) race on the shared counters. */
+int rf_draw_to_bins_noalloc(rf_ctx* ctx, rf_flame* flame, rf_hist* bins,
+                            uint32_t num_iter, uint64_t* drawn);
 /* Density estimation into `out` (device memory of W*H*16 bytes, or NULL to
  * use the histogram's own image buffer), ≙ src/main.cpp:490-515. */
 int rf_density_estimate(rf_ctx* ctx, rf_hist* bins, const rf_de_params* de, void* out_device);

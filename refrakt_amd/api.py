@@ -119,6 +119,7 @@ _SIGS = {
     "rf_draw_to_bins": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_uint64)]),
     "rf_warmup_rows": (C.c_int, [_P, _P, C.c_uint32, C.c_float, C.POINTER(C.c_uint32)]),
     "rf_draw_to_bins_rows": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    "rf_draw_to_bins_noalloc": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_uint64)]),
     "rf_density_estimate": (C.c_int, [_P, _P, C.POINTER(DEParams), _P]),
     "rf_tonemap": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, C.POINTER(TonemapParams)]),
     "rf_density_tonemap": (C.c_int, [_P, _P, C.POINTER(DEParams), C.POINTER(TonemapParams), _P]),
@@ -610,6 +611,13 @@ class FlameHandle:
             assert r.size == 2 * num_iter
             self.ctx.check(self.ctx.L.rf_draw_to_bins_rows(self.ctx.h, self.h, hist.h, num_iter, _uptr(r),
                                                            C.byref(drawn)))
+        return drawn.value
+
+    def draw_to_bins_noalloc(self, hist: "Histogram", num_iter: int) -> int:
+        """rf_draw_to_bins_noalloc: the draw without any device allocation
+        (fast engine, direct histogram modes)."""
+        drawn = C.c_uint64()
+        self.ctx.check(self.ctx.L.rf_draw_to_bins_noalloc(self.ctx.h, self.h, hist.h, num_iter, C.byref(drawn)))
         return drawn.value
 
     def particles(self) -> np.ndarray:

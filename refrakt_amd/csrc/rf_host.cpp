@@ -11,6 +11,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <sstream>
 #include <string>
@@ -797,6 +798,40 @@ static rfk::DEArgs de_args(rf_hist* h, const rf_de_params* de) {
   return a;
 }
 
+// ------------------------------------------------ allocation-free draws
+// rf_draw_to_bins_noalloc keeps its draw counters (the COUNTER_SLOTS x
+// COUNTER_STRIDE u64 block the chaos-game kernels add their binned and
+// LDS-hit counts to) in this file-scope device array instead of the context's
+// buffer.  Every GPU holds its own copy for the life of the process.  In host
+// code `scratch` is only a stand-in for the symbol: kernels get the address
+// hipGetSymbolAddress returned for the device (g_scratch_ptr), which
+// scratch_init takes once per device with that device current.
+constexpr size_t kScratchWords = (size_t)rfk::COUNTER_SLOTS * rfk::COUNTER_STRIDE;
+static_assert(kScratchWords * 8 <= RF_NOALLOC_SCRATCH_BYTES, "draw counters exceed the no-alloc scratch array");
+static __device__ __align__(256) unsigned long long scratch[RF_NOALLOC_SCRATCH_BYTES / 8];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned long long* g_scratch_ptr[kScratchDevices] = {nullptr};
+
+// Takes the address of `scratch` on `dev`, which must be the current device.
+static hipError_t scratch_init(int dev) {
+  if (dev < 0 || dev >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_scratch_ptr[dev] = static_cast<unsigned long long*>(p);
+  return hipSuccess;
+}
+
+static unsigned long long* scratch_ptr(int dev) {
+  if (dev < 0 || dev >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
+
 // Reason of the last failed rf_create (rf_last_error(NULL)).
 static thread_local std::string g_create_err;
 
@@ -858,6 +893,12 @@ int rf_create(const rf_sim_params* p, rf_ctx** out) {
   if (e != hipSuccess) { g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e); delete ctx; return RF_ERR_HIP; }
   e = ctx->counter.alloc(8ull * rfk::COUNTER_SLOTS * rfk::COUNTER_STRIDE);
   if (e != hipSuccess) { g_create_err = std::string("hipMalloc: ") + hipGetErrorString(e); rf_destroy(ctx); return RF_ERR_HIP; }
+  e = scratch_init(p->device);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    rf_destroy(ctx);
+    return RF_ERR_HIP;
+  }
   ctx->ppt = p->total_particles / p->temporal_samples;
   ctx->shuffle_count = ctx->sp.shuffle_count;
   ctx->row_state = p->seed ^ 0x0123456789ABCDEFull;
@@ -1122,6 +1163,56 @@ int rf_draw_to_bins(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t n, uint64_t* 
 int rf_draw_to_bins_rows(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t n, const uint32_t* rows, uint64_t* drawn) {
   if (!rows) return set_err(ctx, RF_ERR_ARG, "rows is null");
   return do_draw(ctx, f, h, n, rows, drawn);
+}
+
+int rf_draw_to_bins_noalloc(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, uint64_t* drawn) {
+  if (!ctx || !f || !h || f->ctx != ctx || h->ctx != ctx) return set_err(ctx, RF_ERR_ARG, "bad context, flame or histogram");
+  if (ctx->sp.engine != RF_ENGINE_FAST) return set_err(ctx, RF_ERR_ARG, "rf_draw_to_bins_noalloc needs the fast engine");
+  if (!f->have_particles || f->generation != ctx->generation || f->needs_update)
+    return set_err(ctx, RF_ERR_STATE, "flame needs a warmup before draw_to_bins");
+  if (ctx->perm_waves != fast_waves(ctx) || !ctx->perms.p)
+    return set_err(ctx, RF_ERR_STATE, "flame needs a warmup before draw_to_bins");
+  unsigned long long* cnt_dev = scratch_ptr(ctx->sp.device);
+  if (!cnt_dev) return set_err(ctx, RF_ERR_STATE, "no scratch address for this device");
+  if (kScratchWords * 8 > RF_NOALLOC_SCRATCH_BYTES)
+    return set_err(ctx, RF_ERR_LIMIT, "draw counters exceed RF_NOALLOC_SCRATCH_BYTES");
+  // The direct histogram modes only: the sort path needs record buffers the
+  // size of the draw.  AUTO runs its current choice, the LDS table in place
+  // of the sort path, and never probes.
+  int hm = ctx->sp.hist_mode == RF_HIST_AUTO ? f->auto_mode : ctx->sp.hist_mode;
+  if (hm == RF_HIST_SORT) hm = RF_HIST_LDS;
+  if (hm == RF_HIST_LDS && !fixed_point_ok(f->desc)) hm = RF_HIST_QUAD;
+  uint64_t host[kScratchWords];
+  HIPCHK(ctx, rfk::launch_zero_u64(cnt_dev, (uint32_t)kScratchWords, ctx->stream));
+  if (num_iter) {
+    const uint64_t P = ctx->sp.total_particles;
+    rfk::FastArgs a = fast_args(ctx, f);
+    a.counter = cnt_dev;
+    a.launch = ctx->launch++;
+    a.n_iter = num_iter;
+    a.first_run = 0; a.do_draw = 1;
+    a.bins = (float*)h->ptr;
+    a.W = h->w; a.H = h->h;
+    screen_affine(f->desc, h->w, h->h, a.ss);
+    const int W = fast_waves(ctx);
+    // a compiled flame program when one is already loaded (compiling would
+    // allocate), else the precompiled interpreter: both give the same bits
+    const rfj::Program* prog = (ctx->programs && !f->host_block.empty()) ?
+                               rfj::find_program(f->host_block[0], {W, hm}) : nullptr;
+    if (prog) HIPCHK(ctx, rfj::launch_fast(prog, a, P, W, hm, ctx->stream));
+    else HIPCHK(ctx, rfk::launch_fast(a, P, W, hm, ctx->stream));
+  }
+  HIPCHK(ctx, hipMemcpyAsync(host, cnt_dev, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  uint64_t n = 0, lds_hits = 0;
+  for (int k = 0; k < rfk::COUNTER_SLOTS; ++k) {
+    n += host[k * rfk::COUNTER_STRIDE];
+    lds_hits += host[k * rfk::COUNTER_STRIDE + 1];
+  }
+  if (hm == RF_HIST_LDS && n) f->last_lds_hit_fraction = (double)lds_hits / (double)n;
+  ctx->last_draw_ms = -1.0;  // not timed: events are created on first use
+  if (drawn) *drawn = n;
+  return RF_OK;
 }
 
 int rf_density_estimate(rf_ctx* ctx, rf_hist* h, const rf_de_params* de, void* out) {

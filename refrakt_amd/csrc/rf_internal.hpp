@@ -76,6 +76,8 @@ constexpr int COUNTER_STRIDE = 16;
 hipError_t launch_init_rs(uint4* rs, uint64_t n, hipStream_t s);
 hipError_t launch_compat_pass(const CompatArgs& a, int hist_mode, hipStream_t s);
 hipError_t launch_fast(const FastArgs& a, uint64_t n_particles, int waves, int hist_mode, hipStream_t s);
+// Fills n u64 with zero (a kernel, not a memset: safe under graph capture).
+hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t s);
 hipError_t launch_animate(rfd::Block* blocks, uint32_t T, float tss_width, const float* rot, hipStream_t s);
 hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s);
 hipError_t launch_tonemap(const float4* in, float4* out, uint32_t n, float gamma, float brightness,

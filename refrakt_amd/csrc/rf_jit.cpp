@@ -248,6 +248,16 @@ const Program* get_program(const rfd::Block& b, const std::vector<std::pair<int,
   return p;
 }
 
+const Program* find_program(const rfd::Block& b, std::pair<int, int> kernel) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  const std::string key = std::to_string(dev) + "/" + structure_key(b);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const auto it = g_cache.find(key);
+  if (it == g_cache.end() || !it->second || !it->second->fns.count(kernel)) return nullptr;
+  return it->second.get();
+}
+
 double program_compile_seconds(const Program* p) { return p ? p->compile_s : 0.0; }
 
 hipError_t launch_batch(const Program* p, const rfk::BatchArgs& a0, int waves, int hist_mode, uint32_t max_grid,

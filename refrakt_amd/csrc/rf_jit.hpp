@@ -52,6 +52,10 @@ bool compile(const std::string& src, const std::string& arch, std::vector<char>&
 const Program* get_program(const rfd::Block& b, const std::vector<std::pair<int, int>>& kernels,
                            std::string& err);
 
+// The already compiled program for b's structure on the current device if it
+// has the entry point `kernel`; nullptr otherwise.  Never compiles or loads.
+const Program* find_program(const rfd::Block& b, std::pair<int, int> kernel);
+
 // Seconds spent compiling the program (0 if it came from the cache).
 double program_compile_seconds(const Program* p);
 

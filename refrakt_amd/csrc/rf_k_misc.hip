@@ -308,6 +308,17 @@ __global__ __launch_bounds__(256) void k_atomic_probe(float* bins, uint64_t nbin
 
 }  // namespace
 
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_animate(Block* blocks, uint32_t T, float tss_width, const float* rot, hipStream_t s) {
   hipLaunchKernelGGL(k_animate, dim3(T), dim3(256), 0, s, blocks, T, tss_width, rot);
   return hipGetLastError();
