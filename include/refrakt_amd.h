@@ -328,8 +328,9 @@ int rf_batch_download(rf_batch* batch, uint32_t i, int32_t which, float* host);
  * normalised log-density image of a low-sample histogram to that of a
  * high-sample one.  Tensors are device memory, NHWC:
  *   x, target: N x H x W x 16 bf16 (channels 0..3 = log10(1 + v * W*H / n)
- *              of the RGBA histogram, bins layout, from rf_log_density_input;
- *              channels 4..15 zero), H and W multiples of 8;
+ *              of the RGBA histogram, bins layout, channel 4 the noise level
+ *              log10(1 + W*H / n), from rf_log_density_input; channels
+ *              5..15 zero), H and W multiples of 8;
  *   y:         N x H x W x 4 float (the denoised log-density).
  * No reference counterpart. */
 typedef struct rf_denoiser rf_denoiser;
@@ -348,7 +349,8 @@ int rf_denoiser_infer(rf_denoiser* dn, const void* x, float* y, int32_t N, int32
 int rf_denoiser_grads(rf_denoiser* dn, const void* x, const void* target, int32_t N, int32_t H, int32_t W,
                       float* grads, float* loss);
 /* x input of the denoiser from a device histogram (npix RGBA floats):
- * out[p, c] = log10(1 + hist[p, c] * scale) for c < 4, 0 for c in 4..15. */
+ * out[p, c] = log10(1 + hist[p, c] * scale) for c < 4,
+ * out[p, 4] = log10(1 + scale), 0 for c in 5..15. */
 int rf_log_density_input(rf_ctx* ctx, const void* hist, void* out, uint32_t npix, double scale);
 /* Kernel-level entry points of the network (tests).  Views: channel stride
  * cs and first channel co of an NHWC bf16 tensor; weights bf16

@@ -3,6 +3,7 @@
  * what each part restates and how it is pinned).  Compiled with
  * -ffp-contract=off so float expressions round exactly as written.
  */
+#define _POSIX_C_SOURCE 200809L
 #include "oracle.h"
 
 #include <math.h>
@@ -636,6 +637,141 @@ void orc_compat_pass(const orc_flame* flames, int stride, uint32_t T, uint32_t p
   }
   free(prep);
   if (drawn) *drawn = cnt;
+}
+
+void orc_jsf_seed_all(uint32_t* out, uint64_t n) {   /* flame.cpp:139-141 for slots 0..n-1 */
+  for (uint64_t i = 0; i < n; ++i) {
+    orc_jsf s;
+    orc_jsf_seed(&s, (uint32_t)i);
+    out[4 * i] = s.a; out[4 * i + 1] = s.b; out[4 * i + 2] = s.c; out[4 * i + 3] = s.d;
+  }
+}
+
+/* ---------------------------------------- reference passes, multithreaded */
+/* The reference's passes (flame.glsl:41-90 as orc_compat_pass restates it),
+ * spread over threads by workgroup, with a float RGBA histogram that every
+ * thread adds to without atomics as the shader does (flame.glsl:82-84: a
+ * read-modify-write that can lose updates when two invocations hit one bin;
+ * relaxed atomic loads and stores keep that race defined in C).  Passes are
+ * separated by a barrier (the glMemoryBarrier after each dispatch,
+ * flame.cpp:262-278, 304-326).  Used as the CPU baseline of the benchmark. */
+typedef struct ref_mt_shared {
+  const orc_flame* f;
+  uint32_t T, ppt, n_passes, W, H;
+  float* pos[2];
+  const uint32_t* shuf;
+  uint32_t* rs;
+  const uint32_t* rows;
+  int random_read, random_write, first_run, do_draw;
+  const float* ss;
+  float* bins;
+  int threads;
+  pthread_barrier_t bar;
+} ref_mt_shared;
+
+typedef struct ref_mt_job { ref_mt_shared* sh; int t; uint64_t drawn; } ref_mt_job;
+
+static void add_relaxed(float* p, float v) {
+  float cur;
+  __atomic_load(p, &cur, __ATOMIC_RELAXED);
+  cur += v;
+  __atomic_store(p, &cur, __ATOMIC_RELAXED);
+}
+
+static void* ref_mt_worker(void* arg) {
+  ref_mt_job* J = (ref_mt_job*)arg;
+  ref_mt_shared* S = J->sh;
+  orc_flame* prep = (orc_flame*)malloc(sizeof(orc_flame));
+  prepare_flame(S->f, prep);
+  const orc_flame* f = prep;
+  const uint32_t groups = S->ppt / 256u, total = S->T * groups;
+  const uint32_t g0 = (uint32_t)((uint64_t)total * J->t / S->threads);
+  const uint32_t g1 = (uint32_t)((uint64_t)total * (J->t + 1) / S->threads);
+  uint64_t cnt = 0;
+  for (uint32_t pass = 0; pass < S->n_passes; ++pass) {
+    const float* pos_in = S->pos[pass % 2];
+    float* pos_out = S->pos[(pass + 1) % 2];
+    const uint32_t in_row = S->rows[2 * pass], out_row = S->rows[2 * pass + 1];
+    const int first = S->first_run && pass == 0;
+    for (uint32_t g = g0; g < g1; ++g) {
+      const uint32_t y = g / groups, wg = g % groups;
+      int xid = 0;
+      for (uint32_t l = 0; l < 256u; ++l) {
+        const uint32_t gx = wg * 256u + l;
+        const size_t slot = (size_t)y * S->ppt + gx;
+        uint32_t* r = S->rs + 4 * slot;
+        orc_jsf st = {r[0], r[1], r[2], r[3]};
+        orc_rng rng = {0, &st, 0, 0};
+        if (l == 0) xid = orc_select(f, orc_jsf_randf(&st));
+        const uint32_t i_idx = S->random_read ? S->shuf[(size_t)in_row * S->ppt + gx] : gx;
+        const uint32_t o_idx = S->random_write ? S->shuf[(size_t)out_row * S->ppt + gx] : gx;
+        const float* p = pos_in + 4 * ((first ? 0 : (size_t)y * S->ppt) + i_idx);
+        float px = p[0], py = p[1], pc = p[2];
+        if (first) {
+          float r1 = orc_jsf_randf(&st), r2 = orc_jsf_randf(&st);
+          float sj = ((r1 * 0.1f) * PI_F) * 2.0f, sn, cs;
+          sincos01(sqrtf(r2), &sn, &cs);
+          px = px + sj * sn;
+          py = py + sj * cs;
+        }
+        float o[4];
+        apply_f32_sorted(&f->xf[xid], px, py, pc, first, &rng, o);
+        float* q = pos_out + 4 * ((size_t)y * S->ppt + o_idx);
+        q[0] = o[0]; q[1] = o[1]; q[2] = o[2]; q[3] = 0.0f;
+        if (S->do_draw) {
+          float rx = o[0], ry = o[1], rc = o[2], rw = o[3];
+          if (f->has_final) {
+            float fq[4];
+            apply_f32_sorted(&f->fin, o[0], o[1], o[2], 0, &rng, fq);
+            rx = fq[0]; ry = fq[1]; rc = fq[2]; rw = fq[3] * o[3];
+          }
+          const float sx = fmaf(S->ss[0], rx, fmaf(S->ss[2], ry, S->ss[4]));
+          const float sy = fmaf(S->ss[1], rx, fmaf(S->ss[3], ry, S->ss[5]));
+          const float fx = floorf(sx), fy = floorf(sy);
+          if (fx >= 0.0f && fy >= 0.0f && fx < (float)S->W && fy < (float)S->H && rw > 0.0f) {
+            const size_t b = (size_t)(S->H - (uint32_t)fy - 1u) * S->W + (uint32_t)fx;
+            const int pi = orc_pal_index(rc);
+            add_relaxed(S->bins + 4 * b + 0, f->pal[pi][0]);
+            add_relaxed(S->bins + 4 * b + 1, f->pal[pi][1]);
+            add_relaxed(S->bins + 4 * b + 2, f->pal[pi][2]);
+            add_relaxed(S->bins + 4 * b + 3, rw);
+            ++cnt;
+          }
+        }
+        r[0] = st.a; r[1] = st.b; r[2] = st.c; r[3] = st.d;
+      }
+    }
+    pthread_barrier_wait(&S->bar);
+  }
+  free(prep);
+  J->drawn = cnt;
+  return 0;
+}
+
+uint64_t orc_ref_passes_mt(const orc_flame* f, uint32_t T, uint32_t ppt, float* pos_a, float* pos_b,
+                           const uint32_t* shuf, uint32_t* rs, const uint32_t* rows, uint32_t n_passes,
+                           int random_read, int random_write, int first_run, int do_draw, uint32_t W, uint32_t H,
+                           const float ss[6], float* bins, int threads) {
+  if (threads < 1) threads = 1;
+  ref_mt_shared S;
+  S.f = f; S.T = T; S.ppt = ppt; S.n_passes = n_passes; S.W = W; S.H = H;
+  S.pos[0] = pos_a; S.pos[1] = pos_b;
+  S.shuf = shuf; S.rs = rs; S.rows = rows;
+  S.random_read = random_read; S.random_write = random_write; S.first_run = first_run; S.do_draw = do_draw;
+  S.ss = ss; S.bins = bins; S.threads = threads;
+  pthread_barrier_init(&S.bar, 0, (unsigned)threads);
+  pthread_t* th = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+  ref_mt_job* jobs = (ref_mt_job*)calloc((size_t)threads, sizeof(ref_mt_job));
+  for (int t = 0; t < threads; ++t) {
+    jobs[t].sh = &S; jobs[t].t = t; jobs[t].drawn = 0;
+    pthread_create(&th[t], 0, ref_mt_worker, &jobs[t]);
+  }
+  uint64_t n = 0;
+  for (int t = 0; t < threads; ++t) { pthread_join(th[t], 0); n += jobs[t].drawn; }
+  pthread_barrier_destroy(&S.bar);
+  free(th);
+  free(jobs);
+  return n;
 }
 
 void orc_fast_run(const orc_flame* flames, int stride, uint32_t ppt, uint64_t P, int waves, const uint16_t* perms,

@@ -107,6 +107,18 @@ void orc_compat_pass(const orc_flame* flames, int stride, uint32_t T, uint32_t p
                      int do_draw, uint32_t W, uint32_t H, const float ss[6],
                      double* bins, uint32_t* hits, uint64_t* drawn);
 
+/* Seeded jsf32 states of slots 0..n-1 (orc_jsf_seed each), n x 4 words. */
+void orc_jsf_seed_all(uint32_t* out, uint64_t n);
+/* n_passes reference passes over P = ppt * T particles on `threads` threads
+ * (workgroups split between threads, a barrier after each pass), ping-pong
+ * between pos_a and pos_b (pass k reads pos[k % 2]), shuffle rows
+ * rows[2k], rows[2k + 1]; bins: W*H*4 floats added without atomics like the
+ * shader (may be NULL when do_draw is 0).  Returns the binned count. */
+uint64_t orc_ref_passes_mt(const orc_flame* f, uint32_t T, uint32_t ppt, float* pos_a, float* pos_b,
+                           const uint32_t* shuf, uint32_t* rs, const uint32_t* rows, uint32_t n_passes,
+                           int random_read, int random_write, int first_run, int do_draw, uint32_t W, uint32_t H,
+                           const float ss[6], float* bins, int threads);
+
 /* This project's fast engine schedule (not a reference algorithm): workgroups
  * of 64*waves particles, one xform per wave per iteration, LDS permutation
  * rows from `perms` (ORC_NPERM rows x 64*waves, row = stream & (ORC_NPERM - 1)). */

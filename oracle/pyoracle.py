@@ -44,6 +44,7 @@ _D = C.POINTER(C.c_double)
 _U = C.POINTER(C.c_uint32)
 _SIGS = {
     "orc_jsf_seed": (None, [C.POINTER(Jsf), C.c_uint32]),
+    "orc_jsf_seed_all": (None, [_U, C.c_uint64]),
     "orc_jsf_next_a": (C.c_uint32, [C.POINTER(Jsf)]),
     "orc_jsf_next_d": (C.c_uint32, [C.POINTER(Jsf)]),
     "orc_jsf_randf": (C.c_float, [C.POINTER(Jsf)]),
@@ -69,6 +70,9 @@ _SIGS = {
     "orc_fast_run": (None, [C.POINTER(OFlame), C.c_int, C.c_uint32, C.c_uint64, C.c_int,
                             C.POINTER(C.c_uint16), C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
                             C.c_uint32, C.c_uint32, _F, _F, _D, _U, C.POINTER(C.c_uint64)]),
+    "orc_ref_passes_mt": (C.c_uint64, [C.POINTER(OFlame), C.c_uint32, C.c_uint32, _F, _F, _U, _U, _U, C.c_uint32,
+                                       C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _F, _F,
+                                       C.c_int]),
     "orc_chaos_f64": (C.c_uint64, [C.POINTER(OFlame), C.c_uint32, C.c_uint32, _D, C.c_uint64, C.c_uint32,
                                    C.c_uint32, C.c_uint64, C.c_int, _D]),
     "orc_density": (None, [_F, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_float, C.c_int, _D]),
@@ -167,12 +171,8 @@ def hammersley(count):
 
 
 def jsf_seed_states(n):
-    L = lib()
     out = np.zeros((n, 4), np.uint32)
-    s = Jsf()
-    for i in range(n):
-        L.orc_jsf_seed(C.byref(s), i)
-        out[i] = (s.a, s.b, s.c, s.d)
+    lib().orc_jsf_seed_all(up(out), n)
     return out
 
 
@@ -231,6 +231,18 @@ def compat_pass(flame_c, T, ppt, pos_in, pos_out, shuf, rs, in_row, out_row, ran
                       dp(bins) if bins is not None else None, up(hits) if hits is not None else None,
                       C.byref(drawn))
     return drawn.value
+
+
+def ref_passes_mt(flame_c, T, ppt, pos_a, pos_b, shuf, rs, rows, n_passes, random_read, random_write, first_run,
+                  do_draw, W=1, H=1, ss=None, bins=None, threads=1):
+    """n_passes reference passes on `threads` threads (orc_ref_passes_mt);
+    pass k reads pos_a if k is even, else pos_b.  Returns the binned count."""
+    L = lib()
+    ss = np.zeros(6, np.float32) if ss is None else np.ascontiguousarray(ss, np.float32)
+    rows = np.ascontiguousarray(rows, np.uint32)
+    return L.orc_ref_passes_mt(C.byref(flame_c), T, ppt, fp(pos_a), fp(pos_b), up(shuf), up(rs), up(rows), n_passes,
+                               int(random_read), int(random_write), int(first_run), int(do_draw), W, H, fp(ss),
+                               fp(bins) if bins is not None else None, int(threads))
 
 
 def fast_run(flame_c, ppt, P, waves, perms, seed, launch, n_iter, first_run, do_draw, W, H, ss, pos, bins=None,

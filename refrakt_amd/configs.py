@@ -57,6 +57,31 @@ def _rot_scale(rng, smin, smax):
     return [c, sn, -sn, c]
 
 
+def random_flame(rng: np.random.Generator, size: int = 512, zoom: float = 1.0) -> Flame:
+    """One flame of the denoiser's family (the cfg2 variation mix): 2-10
+    xforms each mixing 3-5 of the twelve cfg2 variations with U[0, 1]
+    weights normalised, half of the flames with post-affines, half with a
+    final xform, a random smooth palette, square view of side `size` pixels
+    showing [-2.5, 2.5]^2 / zoom."""
+    xs = []
+    post = rng.uniform() < 0.5
+    for _ in range(int(rng.integers(2, 11))):
+        k = int(rng.integers(3, 6))
+        names = list(rng.choice(CFG2_VARIATIONS, size=k, replace=False))
+        w = rng.uniform(0, 1, size=k)
+        xs.append(Xform(affine=_rot_scale(rng, 0.3, 0.9) + list(rng.uniform(-1, 1, 2)),
+                        post=(_rot_scale(rng, 0.8, 1.2) + list(rng.uniform(-0.3, 0.3, 2))) if post else None,
+                        variations={a: float(b) for a, b in zip(names, w / w.sum())},
+                        weight=float(rng.uniform(0.2, 1.0)), color=float(rng.uniform(0, 1)), color_speed=0.5,
+                        opacity=1.0))
+    fin = None
+    if rng.uniform() < 0.5:
+        fin = Xform(affine=_rot_scale(rng, 0.9, 1.1) + [0.0, 0.0], variations={"linear": 1.0}, weight=0.0,
+                    color=float(rng.uniform(0, 1)), color_speed=0.0, opacity=1.0)
+    return Flame(xforms=xs, final_xform=fin, palette=smooth_palette(rng), size=[size, size],
+                 center=[0.0, 0.0], scale=size / 5.0 * zoom, rotate=0.0)
+
+
 @dataclass
 class Workload:
     name: str
@@ -189,4 +214,20 @@ def cfg4(seed: int = 4, frames: int = 64) -> List[Frame]:
     return out
 
 
-WORKLOADS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3}
+# Held-out flames of the denoiser's family (the training set draws from
+# PCG64(2024)), chosen for attractors that fill a good part of the view.
+CFG5_SEEDS = (5030, 5070)
+# Image-quality window of the 2048^2 frame: the central 1024^2 bins, scored
+# at full resolution (pool 1).
+CFG5_WINDOW = (512, 512, 1024, 1024)   # x0, y0, width, height in bins
+
+
+def cfg5(seed: int = CFG5_SEEDS[0]) -> Workload:
+    """Denoiser evaluation frame: one held-out flame of the training family
+    at 2048x2048, 2^22 particles (one pass = 1 sample per pixel), 20
+    unbinned iterations, up to 1024 samples per pixel (16 x 64 passes)."""
+    f = random_flame(np.random.Generator(np.random.PCG64(seed)), 2048)
+    return Workload(f"cfg5_{seed}", f, 2048, 2048, 1 << 22, 1, 19, 64, 16)
+
+
+WORKLOADS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3, "cfg5": cfg5}

@@ -24,6 +24,7 @@ struct rf_batch {
   size_t total_bins = 0;
   DevBuf blocks, palettes, frames, hist, images, perms;
   DevBuf drawn, remaining, done_ts, items, tiles_done, de_done_ts, t0;
+  DevBuf de_rad, de_wt;            // density estimation: radius per bin (largest frame), weight table
   int perm_waves = 0;
   rf_batch_config cfg{8, RF_HIST_LDS, 256, 20};
   double sample_scale = 1.0;       // < 1 while the autotuner times a reduced copy
@@ -141,7 +142,9 @@ int capture(rf_batch* b) {
     g.W = b->W[f];
     g.H = b->H[f];
     g.R = std::max(0, std::min(d.estimator_radius, 100));  // src/main.cpp:502
-    g.rmin = d.estimator_min;
+    g.rmin = std::max(0, std::min(d.estimator_min, 100));
+    g.rad = (signed char*)b->de_rad.p;
+    g.wt = (float*)b->de_wt.p;
     g.curve = d.estimator_curve;
     g.do_tonemap = 1;
     g.gamma = d.gamma;
@@ -227,6 +230,16 @@ int rf_batch_create(rf_ctx* ctx, const rf_frame_desc* frames, uint32_t n, rf_bat
   if (e == hipSuccess) e = b->tiles_done.alloc(4ull * n);
   if (e == hipSuccess) e = b->de_done_ts.alloc(8ull * n);
   if (e == hipSuccess) e = b->t0.alloc(8);
+  size_t max_bins = 0;
+  int span = 0;
+  for (uint32_t f = 0; f < n; ++f) {
+    max_bins = std::max(max_bins, (size_t)b->W[f] * b->H[f]);
+    span = std::max(span, std::max(0, std::min(std::max(b->desc[f].estimator_radius, b->desc[f].estimator_min), 100)));
+  }
+  // frames' DE launches run one after another on the batch's stream, so they
+  // share one radius buffer
+  if (e == hipSuccess) e = b->de_rad.alloc(max_bins);
+  if (e == hipSuccess) e = b->de_wt.alloc(rfk::de_weight_bytes(span));
   if (e == hipSuccess) e = hipMemcpy(b->blocks.p, blk.data(), sizeof(Block) * n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(b->palettes.p, pal.data(), pal.size() * sizeof(float), hipMemcpyHostToDevice);
   if (e != hipSuccess) {

@@ -785,17 +785,23 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
   return RF_OK;
 }
 
-static rfk::DEArgs de_args(rf_hist* h, const rf_de_params* de) {
-  rfk::DEArgs a{};
+// DE arguments for histogram h; grows the histogram's radius buffer and the
+// context's weight table to what the launch needs.
+static int de_args(rf_ctx* ctx, rf_hist* h, const rf_de_params* de, rfk::DEArgs& a) {
+  a = rfk::DEArgs{};
   a.hist = (const float4*)h->ptr;
   a.W = h->w; a.H = h->h;
   if (de) {
-    a.R = std::min(de->estimator_radius, 100);  // src/main.cpp:502
-    if (a.R < 0) a.R = 0;
-    a.rmin = de->estimator_min;
+    a.R = std::max(0, std::min(de->estimator_radius, 100));  // src/main.cpp:502
+    a.rmin = std::max(0, std::min(de->estimator_min, 100));
     a.curve = de->estimator_curve;
   }
-  return a;
+  const size_t nb = (size_t)h->w * h->h, wb = rfk::de_weight_bytes(std::max(a.R, a.rmin));
+  if (h->de_rad.bytes < nb) HIPCHK(ctx, h->de_rad.alloc(nb));
+  if (ctx->de_wt.bytes < wb) HIPCHK(ctx, ctx->de_wt.alloc(wb));
+  a.rad = (signed char*)h->de_rad.p;
+  a.wt = (float*)ctx->de_wt.p;
+  return RF_OK;
 }
 
 // ------------------------------------------------ allocation-free draws
@@ -1218,7 +1224,9 @@ int rf_draw_to_bins_noalloc(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_i
 int rf_density_estimate(rf_ctx* ctx, rf_hist* h, const rf_de_params* de, void* out) {
   if (!ctx || !h || !de) return set_err(ctx, RF_ERR_ARG, "null argument");
   if (!out) { HIPCHK(ctx, h->image.alloc((size_t)h->w * h->h * 16)); out = h->image.p; }
-  rfk::DEArgs a = de_args(h, de);
+  rfk::DEArgs a;
+  int rc = de_args(ctx, h, de, a);
+  if (rc) return rc;
   a.de_out = (float4*)out;
   a.do_tonemap = 0;
   HIPCHK(ctx, rfk::launch_de_tonemap(a, ctx->stream));
@@ -1237,7 +1245,9 @@ int rf_tonemap(rf_ctx* ctx, const void* in, void* out, uint32_t w, uint32_t h, c
 int rf_density_tonemap(rf_ctx* ctx, rf_hist* h, const rf_de_params* de, const rf_tonemap_params* tm, void* out) {
   if (!ctx || !h || !tm) return set_err(ctx, RF_ERR_ARG, "null argument");
   if (!out) { HIPCHK(ctx, h->image.alloc((size_t)h->w * h->h * 16)); out = h->image.p; }
-  rfk::DEArgs a = de_args(h, de);
+  rfk::DEArgs a;
+  int rc = de_args(ctx, h, de, a);
+  if (rc) return rc;
   a.img_out = (float4*)out;
   a.do_tonemap = 1;
   a.gamma = tm->gamma; a.brightness = tm->brightness; a.vibrancy = tm->vibrancy; a.k = tm->scale_constant;
@@ -1275,7 +1285,8 @@ int rf_render(rf_ctx* ctx, rf_flame* f, const rf_render_params* rp, float* hist_
   rf_de_params none{0, 0, 0.0f};
   HIPCHK(ctx, ctx->render_img.alloc((size_t)rp->width * rp->height * 16));
   if (!ctx->ev[0]) { HIPCHK(ctx, hipEventCreate(&ctx->ev[0])); HIPCHK(ctx, hipEventCreate(&ctx->ev[1])); }
-  rfk::DEArgs da = de_args(h, rp->density_estimation ? &rp->de : &none);
+  rfk::DEArgs da;
+  if ((rc = de_args(ctx, h, rp->density_estimation ? &rp->de : &none, da))) return rc;
   da.de_out = (float4*)ctx->render_img.p;
   da.do_tonemap = 0;
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));

@@ -100,6 +100,7 @@ struct rf_ctx {
   rfh::DevBuf probe;           // atomic probe scratch
   rf_hist* render_hist = nullptr;
   rfh::DevBuf render_img;
+  rfh::DevBuf de_wt;           // density estimation: splat weight table
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_draw_ms = 0.0;   // device time of the last draw_to_bins' chaos-game launches
   bool programs = true;        // fast engine: run flame programs (rf_jit.cpp)
@@ -151,4 +152,5 @@ struct rf_hist {
   bool owned = false;
   rfh::DevBuf own;
   rfh::DevBuf image;           // density-estimated / tonemapped output
+  rfh::DevBuf de_rad;          // density estimation: splat radius per bin (W*H bytes)
 };

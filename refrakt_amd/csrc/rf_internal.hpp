@@ -61,7 +61,14 @@ struct DEArgs {
   // finish writes the wall clock to *done_ts; *tiles_done counts workgroups.
   uint32_t* tiles_done;
   unsigned long long* done_ts;
+  // Work buffers: every bin's splat radius (W*H bytes, -1 empty) and the
+  // splat weights of radii 0..max(R, rmin) (de_weight_bytes).
+  signed char* rad;
+  float* wt;
 };
+
+// Bytes of the splat-weight table of radii 0..R (sum over r of (2r+1)^2 floats).
+inline size_t de_weight_bytes(int R) { return (size_t)(4 * (R + 1) * (R + 1) * (R + 1) - (R + 1)) / 3 * 4; }
 
 constexpr int NPERM = 8;          // permutation rows of the fast engine (held in LDS)
 // Draw counters: COUNTER_SLOTS slots of COUNTER_STRIDE u64 (one 128-byte
@@ -79,6 +86,9 @@ hipError_t launch_fast(const FastArgs& a, uint64_t n_particles, int waves, int h
 // Fills n u64 with zero (a kernel, not a memset: safe under graph capture).
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t s);
 hipError_t launch_animate(rfd::Block* blocks, uint32_t T, float tss_width, const float* rot, hipStream_t s);
+// Density estimation (+ tonemap): a.rad (W*H bytes) and a.wt
+// (de_weight_bytes(max(a.R, a.rmin))) must be device buffers the launch may
+// overwrite; R and rmin in [0, 100].
 hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s);
 hipError_t launch_tonemap(const float4* in, float4* out, uint32_t n, float gamma, float brightness,
                           float vibrancy, float k, hipStream_t s);

@@ -63,10 +63,7 @@ __global__ __launch_bounds__(256) void k_animate(Block* blocks, uint32_t T, floa
 // color * (1 - d) * (2/pi)/r^2 where d = ((2j + 1/S)^2 + (2i + 1/S)^2) / S^2,
 // S = 2r + 1, when |i|,|j| <= r and d <= 1; r = 0 copies the bin.
 // r = max(rmin, min(R, int(R / a^curve))).
-//
-// Each workgroup computes a TW x TH tile of texels from an LDS copy of the
-// (TW + 2R) x (TH + 2R) sources around it, then applies the tonemap.
-constexpr int DE_TW = 32, DE_TH = 8;
+// A splat reaches span = max(R, rmin) texels from its centre.
 
 struct DESrc { float r, e, nrm, pad; };
 
@@ -138,50 +135,78 @@ __device__ __forceinline__ void de_stamp(const DEArgs& g) {
   }
 }
 
-// Splat weights of every radius 0..R, computed once per workgroup with
-// de_weight's arithmetic: entry (i + r) * (2r + 1) + (j + r) of radius r
-// starts at de_wt_off(r) = sum over q < r of (2q + 1)^2.
+// Splat weights of every radius 0..R with de_weight's arithmetic: entry
+// (i + r) * (2r + 1) + (j + r) of radius r starts at de_wt_off(r) = sum over
+// q < r of (2q + 1)^2.
 __device__ __forceinline__ int de_wt_off(int r) { return (4 * r * r * r - r) / 3; }
 
-__global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  const int R = g.R;
-  const int SW = DE_TW + 2 * R, SH = DE_TH + 2 * R;
-  float4* s_col = reinterpret_cast<float4*>(smem);
-  // s_m[d * SH + row]: bit c set when the source in tile column c of that row
-  // has a radius of at least d (SW <= 64 columns for R <= 16)
-  unsigned long long* s_m = reinterpret_cast<unsigned long long*>(smem + sizeof(float4) * SW * SH);
-  int* s_r = reinterpret_cast<int*>(s_m + (R + 1) * SH);  // source radius, -1: empty
-  float* s_wt = reinterpret_cast<float*>(s_r + SW * SH);
-  __shared__ int s_rmax;
-  const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * DE_TH;
-  const int tid = threadIdx.x;
-  if (tid == 0) s_rmax = 0;
+__global__ __launch_bounds__(256) void k_de_weights(float* __restrict__ wt, int R) {
   for (int r = 0; r <= R; ++r) {
     const int S = 2 * r + 1;
     DESrc m;
     m.r = (float)r;
     m.e = rf_div_rn(1.0f, (float)S);
     m.nrm = (r > 0) ? rf_div_rn(0.63661977236f, (float)(r * r)) : 1.0f;
-    for (int q = tid; q < S * S; q += DE_TW * DE_TH) s_wt[de_wt_off(r) + q] = de_weight(m, q / S - r, q % S - r);
+    for (int q = threadIdx.x; q < S * S; q += blockDim.x) wt[de_wt_off(r) + q] = de_weight(m, q / S - r, q % S - r);
   }
-  for (int q = tid; q < (R + 1) * SH; q += DE_TW * DE_TH) s_m[q] = 0ull;
+}
+
+// Every bin's splat radius (de_source's arithmetic), -1 for an empty bin.
+__global__ __launch_bounds__(256) void k_de_radius(DEArgs g) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)g.W * g.H) return;
+  const float a = g.hist[i].w;
+  int r = -1;
+  if (a != 0.0f) {
+    const float q = rf_div_rn((float)g.R, powf(a, g.curve));
+    int ri = (q >= (float)g.R) ? g.R : (int)q;
+    if (!(q == q)) ri = g.R;
+    r = ri > g.rmin ? ri : g.rmin;
+    if (r < 0) r = 0;
+  }
+  g.rad[i] = (signed char)r;
+}
+
+// A TW x TH tile of texels per workgroup of 256 threads (TH / 8 texels per
+// thread) from an LDS copy of the (TW + 2R) x (TH + 2R) sources around it.
+// Per source row the tile keeps one 64-bit column mask per radius r (the
+// sources whose radius is exactly r) and a mask of the radii present, so a
+// texel visits, for row offset i, only the radii r >= |i| present in that
+// row and, per radius, only the columns within r of its own: every source
+// it reads contributes (up to the disc test inside the square).
+constexpr int DE_TW = 32;
+
+template <int TH>
+__global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const int R = max(g.R, g.rmin);  // span
+  const int SW = DE_TW + 2 * R, SH = TH + 2 * R;
+  float4* s_col = reinterpret_cast<float4*>(smem);
+  unsigned long long* s_e = reinterpret_cast<unsigned long long*>(smem + sizeof(float4) * SW * SH);  // [r][row]
+  uint32_t* s_rp = reinterpret_cast<uint32_t*>(s_e + (R + 1) * SH);                                    // [row]
+  float* s_wt = reinterpret_cast<float*>(s_rp + SH);
+  __shared__ int s_rmax;
+  const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * TH;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_rmax = 0;
+  for (int q = tid; q < de_wt_off(R + 1); q += 256) s_wt[q] = g.wt[q];
+  for (int q = tid; q < (R + 1) * SH; q += 256) s_e[q] = 0ull;
+  for (int q = tid; q < SH; q += 256) s_rp[q] = 0u;
   __syncthreads();
   int rloc = 0;
-  for (int k = tid; k < SW * SH; k += DE_TW * DE_TH) {
+  for (int k = tid; k < SW * SH; k += 256) {
     const int ly = k / SW, lx = k % SW;
-    float4 col; DESrc m;
-    de_source(g, tx0 - R + lx, ty0 - R + ly, col, m);
-    s_col[k] = col;
-    s_r[k] = (int)m.r;
-    rloc = max(rloc, (int)m.r);
-    for (int d = 0; d <= (int)m.r; ++d) atomicOr(&s_m[d * SH + ly], 1ull << lx);
+    const int cx = tx0 - R + lx, cy = ty0 - R + ly;
+    const int px = cx + 1, py = (int)g.H - 1 - cy;
+    if (cx < 0 || px >= (int)g.W || cy < 0 || cy >= (int)g.H) continue;
+    const size_t b = (size_t)py * g.W + px;
+    const int r = g.rad[b];
+    if (r < 0) continue;
+    s_col[k] = g.hist[b];
+    atomicOr(&s_e[r * SH + ly], 1ull << lx);
+    atomicOr(&s_rp[ly], 1u << r);
+    rloc = max(rloc, r);
   }
-  // A source contributes only within its own radius (de_weight is 0 beyond
-  // it), so offsets beyond the largest radius staged for this tile add
-  // nothing: the loops stop there, and per offset only sources whose radius
-  // reaches it are read.  The sum is unchanged bit for bit (zero-weight terms
-  // were skipped before too).
   rloc = max(rloc, __shfl_xor(rloc, 32));
   rloc = max(rloc, __shfl_xor(rloc, 16));
   rloc = max(rloc, __shfl_xor(rloc, 8));
@@ -191,33 +216,30 @@ __global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
   if ((tid & 63) == 0) atomicMax(&s_rmax, rloc);
   __syncthreads();
   const int RM = min(R, s_rmax);
-  const int lx = tid % DE_TW, ly = tid / DE_TW;
-  const int tx = tx0 + lx, ty = ty0 + ly;
-  if (tx < (int)g.W && ty < (int)g.H) {
+  const int lx = tid % DE_TW;
+#pragma unroll 1
+  for (int ly = tid / DE_TW; ly < TH; ly += 256 / DE_TW) {
+    const int tx = tx0 + lx, ty = ty0 + ly;
+    if (tx >= (int)g.W || ty >= (int)g.H) continue;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    // Only sources whose radius reaches the row offset are visited (the row's
-    // mask for |i|), in the order of the full (i, j) sweep: j ascending is
-    // tile column descending, so the highest set bit comes first.  The terms
-    // left out are the zero ones the sweep skipped, so the sum is the same.
-    const int c_lo = lx + R - RM;  // tile columns c_lo .. c_lo + 2 RM, j = lx + R - c
-    const unsigned long long wmask = (2ull << (2 * RM)) - 1ull;
+    // source (row ly + R - i, column lx + R - j) adds weight w_r(i, j)
     for (int i = -RM; i <= RM; ++i) {
-      // source row cy = ty - i -> LDS row ly + R - i
       const int rowi = ly + R - i;
-      const int row = rowi * SW;
       const int ai = i < 0 ? -i : i;
-      unsigned long long wm = (s_m[ai * SH + rowi] >> c_lo) & wmask;
-      while (wm) {
-        const int b = 63 - __clzll((long long)wm);
-        wm &= ~(1ull << b);
-        const int k = row + c_lo + b;
-        const int j = RM - b;
-        const int r = s_r[k];
-        const int aj = j < 0 ? -j : j;
-        if (r >= aj) {
-          const float wgt = s_wt[de_wt_off(r) + (i + r) * (2 * r + 1) + (j + r)];
+      uint32_t radii = s_rp[rowi] >> ai;
+      while (radii) {
+        const int r = ai + __builtin_ctz(radii);
+        radii &= radii - 1u;
+        const int c0 = lx + R - r;  // columns c0 .. c0 + 2r: j = r .. -r
+        unsigned long long m = (s_e[r * SH + rowi] >> c0) & ((2ull << (2 * r)) - 1ull);
+        const float* w = s_wt + de_wt_off(r) + (i + r) * (2 * r + 1) + 2 * r;  // w[-b]: j = r - b
+        const int base = rowi * SW + c0;
+        while (m) {
+          const int b = __builtin_ctzll(m);
+          m &= m - 1ull;
+          const float wgt = w[-b];
           if (wgt != 0.0f) {
-            const float4 c = s_col[k];
+            const float4 c = s_col[base + b];
             acc.x += c.x * wgt; acc.y += c.y * wgt; acc.z += c.z * wgt; acc.w += c.w * wgt;
           }
         }
@@ -230,12 +252,18 @@ __global__ __launch_bounds__(DE_TW * DE_TH) void k_de_tiled(DEArgs g) {
   de_stamp(g);
 }
 
+template <int TH>
+size_t de_tiled_lds(int R) {
+  const size_t SW = DE_TW + 2 * R, SH = TH + 2 * R;
+  return SW * SH * sizeof(float4) + (size_t)(R + 1) * SH * 8 + SH * 4 + de_weight_bytes(R);
+}
+
 // Radii beyond the LDS tile: the same gather straight from memory.
 __global__ __launch_bounds__(256) void k_de_global(DEArgs g) {
   const int tx = blockIdx.x * 16 + (threadIdx.x % 16);
   const int ty = blockIdx.y * 16 + (threadIdx.x / 16);
   if (tx < (int)g.W && ty < (int)g.H) {
-    const int R = g.R;
+    const int R = max(g.R, g.rmin);
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int i = -R; i <= R; ++i) {
       for (int j = -R; j <= R; ++j) {
@@ -325,13 +353,23 @@ hipError_t launch_animate(Block* blocks, uint32_t T, float tss_width, const floa
 }
 
 hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s) {
-  const int SW = DE_TW + 2 * a.R, SH = DE_TH + 2 * a.R;
-  const size_t n_wt = (size_t)(4 * (a.R + 1) * (a.R + 1) * (a.R + 1) - (a.R + 1)) / 3;  // de_wt_off(R + 1)
-  const size_t sm = (size_t)SW * SH * (sizeof(float4) + sizeof(int)) + (size_t)(a.R + 1) * SH * 8 +
-                    n_wt * sizeof(float);
-  if (sm <= 65536) {  // within the default dynamic-LDS limit of one launch (R <= 14)
-    dim3 grid((a.W + DE_TW - 1) / DE_TW, (a.H + DE_TH - 1) / DE_TH);
-    hipLaunchKernelGGL(k_de_tiled, grid, dim3(DE_TW * DE_TH), sm, s, a);
+  if (!a.rad || !a.wt) return hipErrorInvalidValue;
+  const uint64_t n = (uint64_t)a.W * a.H;
+  // Tiles of 32 x 32 texels while they fit the 64 KiB of LDS one launch gets
+  // by default (R <= 11), 32 x 8 up to R = 14 (64-column masks need
+  // 32 + 2R <= 64 as well), the global-memory gather beyond.
+  const int span = a.R > a.rmin ? a.R : a.rmin;
+  const bool tall = de_tiled_lds<32>(span) <= 65536, flat = de_tiled_lds<8>(span) <= 65536 && 32 + 2 * span <= 64;
+  if (tall || flat) {
+    hipLaunchKernelGGL(k_de_weights, dim3(1), dim3(256), 0, s, a.wt, span);
+    if (n) hipLaunchKernelGGL(k_de_radius, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  }
+  if (tall) {
+    dim3 grid((a.W + DE_TW - 1) / DE_TW, (a.H + 31) / 32);
+    hipLaunchKernelGGL(k_de_tiled<32>, grid, dim3(256), de_tiled_lds<32>(span), s, a);
+  } else if (flat) {
+    dim3 grid((a.W + DE_TW - 1) / DE_TW, (a.H + 7) / 8);
+    hipLaunchKernelGGL(k_de_tiled<8>, grid, dim3(256), de_tiled_lds<8>(span), s, a);
   } else {
     dim3 grid((a.W + 15) / 16, (a.H + 15) / 16);
     hipLaunchKernelGGL(k_de_global, grid, dim3(256), 0, s, a);

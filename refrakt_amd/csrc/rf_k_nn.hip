@@ -616,11 +616,16 @@ __global__ void k_to_bf16(const float* __restrict__ in, bf16* __restrict__ out, 
   if (i < n) out[i] = f2bf(in[i]);
 }
 
+// Network input: channels 0-3 the normalised log-density log10(1 + v scale)
+// of the bin's RGBA, channel 4 the noise level log10(1 + scale) (what one
+// sample in a bin reads: scale = W H / n is the inverse samples per pixel),
+// channels 5-15 zero.
 __global__ void k_log_density(const float* __restrict__ hist, bf16* __restrict__ out, int npix, float scale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npix * 16) return;
   const int p = i >> 4, c = i & 15;
-  out[i] = c < 4 ? f2bf(log10f(1.0f + hist[(size_t)p * 4 + c] * scale)) : (bf16)0;
+  out[i] = c < 4 ? f2bf(log10f(1.0f + hist[(size_t)p * 4 + c] * scale))
+                 : (c == 4 ? f2bf(log10f(1.0f + scale)) : (bf16)0);
 }
 
 // The same values, one pixel per thread: one 16-byte load of the bin and two
@@ -631,7 +636,8 @@ __global__ void k_log_density_v(const float4* __restrict__ hist, uint4* __restri
   const float4 h = hist[p];
   const uint32_t lo = (uint32_t)f2bf(log10f(1.0f + h.x * scale)) | ((uint32_t)f2bf(log10f(1.0f + h.y * scale)) << 16);
   const uint32_t hi = (uint32_t)f2bf(log10f(1.0f + h.z * scale)) | ((uint32_t)f2bf(log10f(1.0f + h.w * scale)) << 16);
-  out[2 * (size_t)p] = make_uint4(lo, hi, 0u, 0u);
+  const uint32_t nl = (uint32_t)f2bf(log10f(1.0f + scale));
+  out[2 * (size_t)p] = make_uint4(lo, hi, nl, 0u);
   out[2 * (size_t)p + 1] = make_uint4(0u, 0u, 0u, 0u);
 }
 

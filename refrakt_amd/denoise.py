@@ -6,8 +6,9 @@ log10(1 + v * W * H / n) per RGBA channel (bins layout), to that of a
 high-sample histogram of the same flame.  Training pairs are rendered by the
 batched renderer: random flames of 2-10 xforms mixing 3-5 of the twelve
 variations {linear ... bubble} (half of them with post-affines, half with a
-final xform) at 512x512, 4 samples per pixel for the input and 1024 for the
-target.  The network, its kernels and Adam live in the library
+final xform) at 512x512 and zoom 1, 2 or 4, 1-32 samples per pixel for the
+input (log-uniform; the input carries its noise level in channel 4) and 1024
+for the target.  The network, its kernels and Adam live in the library
 (rf_denoiser_*); PyTorch only holds the device tensors of the data set.
 
     python -m refrakt_amd.denoise --flames 2000 --steps 3000 --out weights.npz
@@ -24,46 +25,37 @@ from typing import List, Optional
 
 import numpy as np
 
-from .api import RF_HIST_LDS, Context, Flame, Xform
-from .configs import CFG2_VARIATIONS, _rot_scale, smooth_palette
+from .api import RF_HIST_LDS, Context, Flame
+from .configs import random_flame
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "denoiser_weights.npz")
 
 
-def training_flames(n: int, seed: int = 2024, size: int = 512) -> List[Flame]:
+def training_flames(n: int, seed: int = 2024, size: int = 512, zooms=(1.0,)) -> List[Flame]:
+    """n flames of the family; flame i is shown at zoom zooms[i % len(zooms)]
+    (zoom 4 at 512^2 has the pixel scale of the whole view at 2048^2)."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    out = []
-    for _ in range(n):
-        xs = []
-        post = rng.uniform() < 0.5
-        for _ in range(int(rng.integers(2, 11))):
-            k = int(rng.integers(3, 6))
-            names = list(rng.choice(CFG2_VARIATIONS, size=k, replace=False))
-            w = rng.uniform(0, 1, size=k)
-            xs.append(Xform(affine=_rot_scale(rng, 0.3, 0.9) + list(rng.uniform(-1, 1, 2)),
-                            post=(_rot_scale(rng, 0.8, 1.2) + list(rng.uniform(-0.3, 0.3, 2))) if post else None,
-                            variations={a: float(b) for a, b in zip(names, w / w.sum())},
-                            weight=float(rng.uniform(0.2, 1.0)), color=float(rng.uniform(0, 1)), color_speed=0.5,
-                            opacity=1.0))
-        fin = None
-        if rng.uniform() < 0.5:
-            fin = Xform(affine=_rot_scale(rng, 0.9, 1.1) + [0.0, 0.0], variations={"linear": 1.0}, weight=0.0,
-                        color=float(rng.uniform(0, 1)), color_speed=0.0, opacity=1.0)
-        out.append(Flame(xforms=xs, final_xform=fin, palette=smooth_palette(rng), size=[size, size],
-                         center=[0.0, 0.0], scale=size / 5.0, rotate=0.0))
-    return out
+    return [random_flame(rng, size, zooms[i % len(zooms)]) for i in range(n)]
 
 
-def render_log_density(ctx: Context, flames: List[Flame], W: int, H: int, spp: float, out, iters: int,
+def training_spp(n: int, seed: int, lo: float, hi: float) -> np.ndarray:
+    """Input sample densities of n training pairs: log-uniform on [lo, hi]."""
+    rng = np.random.Generator(np.random.PCG64(seed ^ 0x5bb))
+    return np.exp(rng.uniform(math.log(lo), math.log(hi), size=n))
+
+
+def render_log_density(ctx: Context, flames: List[Flame], W: int, H: int, spp, out, iters: int,
                        chunk: int = 200):
-    """Renders every flame at spp samples per pixel with the batched renderer
-    and writes the denoiser input (16-channel bf16 log-density) into
-    out[i] (a torch tensor N x H x W x 16, bfloat16, on the GPU)."""
+    """Renders every flame at spp samples per pixel (a number, or one per
+    flame) with the batched renderer and writes the denoiser input (16-channel
+    bf16 log-density and noise level) into out[i] (a torch tensor
+    N x H x W x 16, bfloat16, on the GPU)."""
     import torch
+    spps = list(spp) if hasattr(spp, "__len__") else [spp] * len(flames)
     for c0 in range(0, len(flames), chunk):
         part = flames[c0:c0 + chunk]
-        b = ctx.batch([(f, W, H, int(spp * W * H)) for f in part])
+        b = ctx.batch([(f, W, H, max(1, int(s * W * H))) for f, s in zip(part, spps[c0:c0 + chunk])])
         b.set_config(8, RF_HIST_LDS, iters, 20)
         b.run()
         drawn = b.frame_stats()["drawn"]
@@ -97,16 +89,22 @@ def denoise_histogram(ctx: Context, dn, hist_ptr: int, W: int, H: int, n: int, x
     return y[0]
 
 
+ZOOMS = (1.0, 2.0, 4.0)
+
+
 def train(n_flames: int = 2000, steps: int = 3000, batch: int = 8, crop: int = 256, lr: float = 1e-3,
           seed: int = 2024, out: Optional[str] = None, log_every: int = 100, size: int = 512,
-          spp_lo: float = 4.0, spp_hi: float = 1024.0, val_flames: int = 64):
+          spp_lo: float = 1.0, spp_in_hi: float = 32.0, spp_hi: float = 1024.0, val_flames: int = 64):
+    """Pairs: each flame (zoom 1, 2 or 4) rendered at an input density drawn
+    log-uniformly from [spp_lo, spp_in_hi] samples per pixel and at spp_hi."""
     import torch
     ctx = Context(1 << 12, 1, 16, seed=seed)
-    flames = training_flames(n_flames + val_flames, seed, size)
+    flames = training_flames(n_flames + val_flames, seed, size, ZOOMS)
+    spp_in = training_spp(len(flames), seed, spp_lo, spp_in_hi)
     t0 = time.time()
     X = torch.empty((len(flames), size, size, 16), dtype=torch.bfloat16, device="cuda")
     T = torch.empty_like(X)
-    render_log_density(ctx, flames, size, size, spp_lo, X, 64)
+    render_log_density(ctx, flames, size, size, spp_in, X, 64)
     render_log_density(ctx, flames, size, size, spp_hi, T, 1024)
     t_data = time.time() - t0
     print(json.dumps({"event": "data", "flames": len(flames), "seconds": round(t_data, 1)}), flush=True)
@@ -135,17 +133,21 @@ def train(n_flames: int = 2000, steps: int = 3000, batch: int = 8, crop: int = 2
     t_train = time.time() - t0
     # validation: held-out flames, full frames, MSE of the input and of the output
     y = torch.empty((1, size, size, 4), dtype=torch.float32, device="cuda")
-    mse_in, mse_out = [], []
+    mse_in, mse_out, by_spp = [], [], {}
     for i in range(n_flames, len(flames)):
         dn.infer(X[i:i + 1].data_ptr(), y.data_ptr(), 1, size, size)
         t = T[i, :, :, :4].float()
         mse_in.append(float(((X[i, :, :, :4].float() - t) ** 2).mean()))
         mse_out.append(float(((y[0] - t) ** 2).mean()))
-    meta = {"flames": n_flames, "val_flames": val_flames, "size": size, "spp_in": spp_lo, "spp_target": spp_hi,
+        band = "1-4" if spp_in[i] < 4 else ("4-16" if spp_in[i] < 16 else "16-32")
+        by_spp.setdefault(band, []).append(10.0 * math.log10(mse_in[-1] / max(mse_out[-1], 1e-12)))
+    meta = {"flames": n_flames, "val_flames": val_flames, "size": size, "spp_in": [spp_lo, spp_in_hi],
+            "zooms": list(ZOOMS), "spp_target": spp_hi, "input_channels": "log-density RGBA + noise level",
             "steps": steps, "batch": batch, "crop": crop, "lr": lr, "seed": seed,
             "data_seconds": round(t_data, 1), "train_seconds": round(t_train, 1),
             "final_loss": hist[-1][1] if hist else None,
             "val_mse_input": float(np.mean(mse_in)), "val_mse_output": float(np.mean(mse_out)),
+            "val_gain_db_by_spp": {k: round(float(np.mean(v)), 2) for k, v in sorted(by_spp.items())},
             "n_params": dn.n_params}
     print(json.dumps({"event": "done", **meta}), flush=True)
     if out:

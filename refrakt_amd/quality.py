@@ -24,12 +24,14 @@ def pool_factor(width: int, max_width: int = 512) -> int:
     return f
 
 
-def log_density(rgba, n_binned: int, pool: int):
-    """Normalised log-density image, (H // pool, W // pool, 4)."""
+def log_density(rgba, n_binned: int, pool: int, frame_pixels: int = 0):
+    """Normalised log-density image, (H // pool, W // pool, 4).  For a window
+    cut from a larger frame, frame_pixels is the frame's W * H (the
+    normalisation is the frame's mean density)."""
     H, W = rgba.shape[0], rgba.shape[1]
     h, w = H // pool, W // pool
     x = rgba[:h * pool, :w * pool].reshape(h, pool, w, pool, 4)
-    scale = (W * H) / float(n_binned)
+    scale = float(frame_pixels or W * H) / float(n_binned)
     if hasattr(x, "double") and not hasattr(x, "astype"):  # torch
         import torch
         m = x.double().mean(dim=(1, 3)) * scale

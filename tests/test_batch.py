@@ -154,3 +154,43 @@ def test_batch_with_mixed_structures_is_interpreted(rf, gpu_available):
     b.run()
     assert b.program_info()[0] == 0
     assert (b.frame_stats()["drawn"] > 0).all()
+
+
+def test_cfg4_workload_against_fp64_oracle(rf, gpu_available):
+    """The cfg4 workload itself (64 animation frames of one flame family, 2-12
+    xforms, 512^2 to 2048^2, 64-512 samples per pixel) in one graph replay:
+    every frame's alpha sum equals its drawn count, and three frames (512^2 /
+    12 xforms at 139 spp, 1024^2 / 10 xforms at 374 spp, 1920x1080 / 11
+    xforms at 124 spp) score at least 40 dB against the FP64 oracle's
+    log-density image at about 1024 spp (tests/golden/psnr_cfg4.npz; pool 2,
+    pool 4 for the 1920-wide frame)."""
+    import json
+    import os
+
+    import torch
+    from refrakt_amd import configs
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    meta = json.load(open(os.path.join(golden, "psnr_cfg4.json")))
+    refs = np.load(os.path.join(golden, "psnr_cfg4.npz"))
+    frames = configs.cfg4()
+    ctx = rf.Context(1 << 12, 1, 16, seed=12345)
+    b = ctx.batch([(f.flame, f.width, f.height, f.samples) for f in frames])
+    print("cfg4: 64 frames, one replay", flush=True)
+    b.run()
+    st = b.frame_stats()
+    for i, f in enumerate(frames):
+        h = b.histogram(i)
+        d = int(st["drawn"][i])
+        assert d >= 0.5 * f.samples, (i, d, f.samples)
+        assert int(h[..., 3].astype(np.float64).sum()) == d, i
+        fm = meta["frames"].get(str(i))
+        if fm is not None:
+            ref = quality.dequantize(refs[f"f{i}"], fm["peak_raw"])
+            p = quality.psnr(quality.log_density(h, d, fm["pool"]), ref)
+            print(f"cfg4 frame {i} ({f.width}x{f.height}, {len(f.flame.xforms)} xforms, "
+                  f"{d / (f.width * f.height):.0f} spp): {p:.2f} dB vs FP64", flush=True)
+            assert p >= 40.0, (i, p)
+    assert sorted(int(k) for k in meta["frames"]) == [3, 10, 11]
+    b.close()
+    ctx.close()
+    torch.cuda.synchronize()

@@ -1,5 +1,7 @@
 """Denoiser network kernels and the U-Net against PyTorch fp32 references
 (computed on the CPU from the same bf16-rounded inputs and weights)."""
+import math
+
 import numpy as np
 import pytest
 
@@ -124,7 +126,8 @@ def test_conv3x3_gradients_match_torch(rf, gpu_available, Cin, Cout, W):
 
 @pytest.mark.parametrize("npix", [1, 257, 4096 + 3])
 def test_log_density_input_matches_formula(rf, gpu_available, npix):
-    """bf16(log10(1 + v scale)) in channels 0..3, zeros in 4..15; the
+    """bf16(log10(1 + v scale)) in channels 0..3, the noise level
+    bf16(log10(1 + scale)) in channel 4, zeros in 5..15; the
     one-pixel-per-thread kernel (16-byte aligned buffers) and the per-value
     kernel (an output 2 bytes off alignment) give the same bits."""
     import torch
@@ -141,7 +144,9 @@ def test_log_density_input_matches_formula(rf, gpu_available, npix):
     a = out_a.cpu().view(npix, 16)
     b = out_b[1:].cpu().view(npix, 16)
     assert torch.equal(a.view(torch.int16), b.view(torch.int16))
-    assert (a[:, 4:] == 0).all()
+    assert (a[:, 5:] == 0).all()
+    nl = math.log10(1.0 + float(np.float32(scale)))
+    assert (torch.abs(a[:, 4].double() - nl) <= 2.0 ** -8 * nl).all()
     ref = torch.log10(1.0 + h.double() * np.float32(scale))
     err = (a[:, :4].double() - ref).abs()
     assert (err <= 2.0 ** -8 * ref.abs() + 1e-6).all(), float(err.max())

@@ -277,7 +277,10 @@ def _sparse_hist(rng, W, H, density=0.3):
     return h
 
 
-@pytest.mark.parametrize("R,rmin,curve", [(9, 0, 0.4), (3, 1, 0.6), (0, 0, 0.4), (30, 0, 0.5)])
+# R <= 11: 32 x 32 tiles; 12-16: 32 x 8 tiles; beyond: the global gather;
+# rmin > R: the splat span is rmin
+@pytest.mark.parametrize("R,rmin,curve", [(9, 0, 0.4), (3, 1, 0.6), (0, 0, 0.4), (30, 0, 0.5), (11, 0, 0.3),
+                                          (13, 0, 0.45), (2, 5, 0.4)])
 def test_density_estimation_vs_oracle(rf, orc, gpu_available, R, rmin, curve):
     import torch
     rng = np.random.default_rng(R + 17)

@@ -270,6 +270,59 @@ def test_compat_draw_counts_match_positions(orc):
     assert np.array_equal(hits, ref)
 
 
+@pytest.mark.parametrize("threads", [1, 3])
+def test_ref_passes_mt_equals_compat_pass(orc, threads):
+    """orc_ref_passes_mt (the CPU baseline: the reference's passes spread over
+    threads, float histogram) computes orc_compat_pass's positions, jsf32
+    states and drawn counts bit for bit; on one thread its histogram holds
+    exactly the compat pass's hit counts; on several, adds to one bin may race
+    as in the shader (flame.glsl:82-84), so alpha may only fall short."""
+    from refrakt_amd.api import Flame, Xform
+    xs = [Xform(affine=[0.5, 0.1, -0.2, 0.6, 0.3 * k - 0.3, 0.2], variations={"linear": 0.7, "spherical": 0.3},
+                weight=1.0 + k, color=k / 3.0, color_speed=0.5, opacity=1.0) for k in range(3)]
+    pal = np.ones((256, 4), np.float32)
+    pal[:, 0] = np.linspace(0, 1, 256)
+    fl = Flame(xforms=xs, palette=pal, size=[96, 80], scale=30.0)
+    T, ppt, W, H = 4, 2048, 96, 80
+    P = T * ppt
+    fc = orc.flame_to_c(fl)
+    rng = np.random.default_rng(4)
+    R = 6
+    shuf = np.stack([rng.permutation(ppt).astype(np.uint32) for _ in range(R)])
+    rows = rng.integers(0, R, size=2 * 10).astype(np.uint32)
+    ss = orc.screen_affine(fl, W, H)
+    samples = orc.hammersley(ppt)
+    # single-threaded oracle pass by pass
+    rs1 = orc.jsf_seed_states(P)
+    a, b = np.zeros((P, 4), np.float32), np.zeros((P, 4), np.float32)
+    orc.compat_pass(fc, T, ppt, samples, a, shuf, rs1, rows[0], rows[1], 1, 1, 1, 0)
+    bufs = [a, b]
+    for i in range(5):
+        orc.compat_pass(fc, T, ppt, bufs[i % 2], bufs[(i + 1) % 2], shuf, rs1, rows[2 + 2 * i], rows[3 + 2 * i],
+                        1, 1, 0, 0)
+    hits = np.zeros((H, W), np.uint32)
+    d1 = 0
+    for i in range(4):
+        d1 += orc.compat_pass(fc, T, ppt, bufs[(5 + i) % 2], bufs[(6 + i) % 2], shuf, rs1, rows[12 + 2 * i],
+                              rows[13 + 2 * i], 1, 0, 0, 1, W, H, ss, None, hits)
+    final1 = bufs[(5 + 4) % 2]
+    # the multithreaded restatement
+    rs2 = orc.jsf_seed_states(P)
+    A, B = np.zeros((P, 4), np.float32), np.zeros((P, 4), np.float32)
+    orc.ref_passes_mt(fc, T, ppt, samples, A, shuf, rs2, rows[:2], 1, 1, 1, 1, 0, threads=threads)
+    orc.ref_passes_mt(fc, T, ppt, A, B, shuf, rs2, rows[2:12], 5, 1, 1, 0, 0, threads=threads)
+    bins = np.zeros((H, W, 4), np.float32)
+    d2 = orc.ref_passes_mt(fc, T, ppt, B, A, shuf, rs2, rows[12:20], 4, 1, 0, 0, 1, W, H, ss, bins, threads)
+    final2 = B   # 5 warmup passes end in B; 4 draw passes B->A->B->A->B
+    assert d1 == d2 > 0
+    assert np.array_equal(rs1, rs2)
+    assert np.array_equal(final1.view(np.uint32), final2.view(np.uint32))
+    if threads == 1:
+        assert np.array_equal(bins[..., 3].astype(np.uint32), hits)
+    else:
+        assert (bins[..., 3] <= hits + 0.5).all() and bins[..., 3].sum() >= 0.99 * d2
+
+
 # --------------------------------------------------------- chaos game (f64)
 def test_chaos_f64_sierpinski_mass(orc):
     """Three maps p -> (p + v_i)/2: the attractor is the Sierpinski triangle,
