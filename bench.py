@@ -33,8 +33,6 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 KERNELS = {1: "k_fast<quad>", 2: "k_fast<wave-aggregated>", 3: "k_fast<naive>", 4: "k_fast<LDS bin cache>",
            5: "k_fast<sort staging> + k_sort_rebucket + k_sort_tiles"}
-# kernels of one draw, by entry-point name prefix (flame programs are rf_prog_w<waves>_h<mode>)
-DRAW_KERNELS = ("rf_prog_", "k_fast", "k_sort_reset", "k_sort_rebucket", "k_sort_tiles")
 CFG5_SPP = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024]
 COPY_BYTES = 1 << 29
 
@@ -170,6 +168,13 @@ def _counter_rows(d):
         return list(csv.DictReader(fh))
 
 
+def _short_name(name):
+    """Entry-point name of a (possibly demangled, namespaced) kernel name."""
+    import re
+    m = re.search(r"(rf_prog_w\d+_h\d+|k_\w+|__amd_rocclr_\w+)", name)
+    return m.group(1) if m else name
+
+
 def measure_traffic(args, timeout=180):
     """HBM bytes of one steady-state draw from two rocprofv3 --pmc runs of
     `bench.py --traffic-child` (FETCH_SIZE, then WRITE_SIZE, KiB per
@@ -193,14 +198,22 @@ def measure_traffic(args, timeout=180):
         shutil.rmtree(d, ignore_errors=True)
         if not rows:
             return None, {"error": f"{ctr}: no counter output (rc {r.returncode})", "tail": r.stdout[-400:]}
-        vals = [(int(x.get("Dispatch_Id", 0)), x.get("Kernel_Name", ""), float(x.get("Counter_Value", 0.0)))
-                for x in rows if x.get("Counter_Name", ctr) == ctr]
+        vals = [(int(x.get("Dispatch_Id", 0)), _short_name(x.get("Kernel_Name", "")),
+                 float(x.get("Counter_Value", 0.0))) for x in rows if x.get("Counter_Name", ctr) == ctr]
         vals.sort()
         copies = [v for _, n, v in vals if "copyBuffer" in n and v * 1024 > COPY_BYTES / 8]
+        # the last draw: its sort-reset (if any), the last chaos-game launch and
+        # the record passes after it
+        chaos = [k for k, (_, n, _) in enumerate(vals) if n.startswith(("rf_prog_", "k_fast"))]
         last = {}
-        for _, n, v in vals:
-            if n.startswith(DRAW_KERNELS) or any(k in n for k in DRAW_KERNELS):
-                last[n.split("(")[0].split("<")[0].strip()] = v
+        if chaos:
+            k0 = chaos[-1]
+            if k0 > 0 and vals[k0 - 1][1] == "k_sort_reset":
+                last["k_sort_reset"] = vals[k0 - 1][2]
+            last[vals[k0][1]] = vals[k0][2]
+            for _, n, v in vals[k0 + 1:]:
+                if n in ("k_sort_rebucket", "k_sort_tiles"):
+                    last[n] = v
         if len(copies) < 3 or not last:
             return None, {"error": f"{ctr}: copies {len(copies)}, draw kernels {sorted(last)}"}
         factor = COPY_BYTES / (np.mean(copies[:3]) * 1024.0)

@@ -185,6 +185,13 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
   unsigned long long* s_e = reinterpret_cast<unsigned long long*>(smem + sizeof(float4) * SW * SH);  // [r][row]
   uint32_t* s_rp = reinterpret_cast<uint32_t*>(s_e + (R + 1) * SH);                                    // [row]
   float* s_wt = reinterpret_cast<float*>(s_rp + SH);
+  // s_jx[r * (2R + 1) + i + R]: the columns j with a non-zero weight in row
+  // offset i of radius r form the interval [lo, hi] (the disc's chord);
+  // .x = (R - hi) | (hi - lo) << 8 (mask shift relative to the texel's
+  // column and chord length - 1), .y = index of w(i, hi) in s_wt; .x = ~0u
+  // when the row has none
+  const size_t jx_off = ((size_t)(reinterpret_cast<unsigned char*>(s_wt + de_wt_off(R + 1)) - smem) + 7) & ~(size_t)7;
+  uint2* s_jx = reinterpret_cast<uint2*>(smem + jx_off);
   __shared__ int s_rmax;
   const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * TH;
   const int tid = threadIdx.x;
@@ -192,6 +199,21 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
   for (int q = tid; q < de_wt_off(R + 1); q += 256) s_wt[q] = g.wt[q];
   for (int q = tid; q < (R + 1) * SH; q += 256) s_e[q] = 0ull;
   for (int q = tid; q < SH; q += 256) s_rp[q] = 0u;
+  for (int q = tid; q < (R + 1) * (2 * R + 1); q += 256) {
+    const int r = q / (2 * R + 1), i = q % (2 * R + 1) - R;
+    uint32_t v;
+    if (i >= -r && i <= r) {
+      const float* w = g.wt + de_wt_off(r) + (i + r) * (2 * r + 1);
+      int lo = r + 1, hi = -r - 1;
+      for (int j = -r; j <= r; ++j)
+        if (w[j + r] != 0.0f) { lo = min(lo, j); hi = max(hi, j); }
+      if (lo <= hi) v = (uint32_t)(R - hi) | ((uint32_t)(hi - lo) << 8);
+      else v = ~0u;
+      s_jx[q] = make_uint2(v, (uint32_t)(de_wt_off(r) + (i + r) * (2 * r + 1) + hi + r));
+    } else {
+      s_jx[q] = make_uint2(~0u, 0u);
+    }
+  }
   __syncthreads();
   int rloc = 0;
   for (int k = tid; k < SW * SH; k += 256) {
@@ -230,18 +252,22 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
       while (radii) {
         const int r = ai + __builtin_ctz(radii);
         radii &= radii - 1u;
-        const int c0 = lx + R - r;  // columns c0 .. c0 + 2r: j = r .. -r
-        unsigned long long m = (s_e[r * SH + rowi] >> c0) & ((2ull << (2 * r)) - 1ull);
-        const float* w = s_wt + de_wt_off(r) + (i + r) * (2 * r + 1) + 2 * r;  // w[-b]: j = r - b
+        const uint2 jx = s_jx[r * (2 * R + 1) + i + R];
+        if (jx.x == ~0u) continue;
+        const int c0 = lx + (int)(jx.x & 0xffu);  // columns c0 .. c0 + hi - lo: j = hi .. lo
+        // a chord is at most 2r + 1 <= 29 columns wide: the mask fits 32 bits
+        uint32_t m = (uint32_t)(s_e[r * SH + rowi] >> c0) & ((2u << (jx.x >> 8)) - 1u);
+        const float* w = s_wt + jx.y;  // w[-b]: j = hi - b
         const int base = rowi * SW + c0;
         while (m) {
-          const int b = __builtin_ctzll(m);
-          m &= m - 1ull;
+          const int b = __builtin_ctz(m);
+          m &= m - 1u;
           const float wgt = w[-b];
-          if (wgt != 0.0f) {
-            const float4 c = s_col[base + b];
-            acc.x += c.x * wgt; acc.y += c.y * wgt; acc.z += c.z * wgt; acc.w += c.w * wgt;
-          }
+          const float4 c = s_col[base + b];
+          acc.x = fmaf(c.x, wgt, acc.x);
+          acc.y = fmaf(c.y, wgt, acc.y);
+          acc.z = fmaf(c.z, wgt, acc.z);
+          acc.w = fmaf(c.w, wgt, acc.w);
         }
       }
     }
@@ -255,7 +281,8 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
 template <int TH>
 size_t de_tiled_lds(int R) {
   const size_t SW = DE_TW + 2 * R, SH = TH + 2 * R;
-  return SW * SH * sizeof(float4) + (size_t)(R + 1) * SH * 8 + SH * 4 + de_weight_bytes(R);
+  return SW * SH * sizeof(float4) + (size_t)(R + 1) * SH * 8 + SH * 4 + de_weight_bytes(R) + 8 +
+         (size_t)(R + 1) * (2 * R + 1) * 8;
 }
 
 // Radii beyond the LDS tile: the same gather straight from memory.
@@ -359,7 +386,9 @@ hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s) {
   // by default (R <= 11), 32 x 8 up to R = 14 (64-column masks need
   // 32 + 2R <= 64 as well), the global-memory gather beyond.
   const int span = a.R > a.rmin ? a.R : a.rmin;
-  const bool tall = de_tiled_lds<32>(span) <= 65536, flat = de_tiled_lds<8>(span) <= 65536 && 32 + 2 * span <= 64;
+  // (chords of at most 31 columns keep the tap masks in 32 bits)
+  const bool tall = de_tiled_lds<32>(span) <= 65536 && span <= 15;
+  const bool flat = de_tiled_lds<8>(span) <= 65536 && 32 + 2 * span <= 64 && span <= 15;
   if (tall || flat) {
     hipLaunchKernelGGL(k_de_weights, dim3(1), dim3(256), 0, s, a.wt, span);
     if (n) hipLaunchKernelGGL(k_de_radius, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
