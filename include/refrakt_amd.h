@@ -272,9 +272,12 @@ int rf_render(rf_ctx* ctx, rf_flame* flame, const rf_render_params* params,
               float* hist_out, float* image_out, rf_stats* stats);
 
 /* ---------------------------------------------------------- batching */
-/* Many frames (each its own flame, size and sample count) rendered by one
- * persistent chaos-game launch followed by every frame's DE + tonemap, all
- * captured once in a HIP graph and replayed by rf_batch_run.  This
+/* Many frames (each its own flame, size and sample count) rendered in one
+ * HIP graph replayed by rf_batch_run: the work items (fresh particles of one
+ * frame) run in frame order in slices of at most 2^29 samples, each one
+ * persistent chaos-game launch (with RF_HIST_SORT followed by its record
+ * passes); the DE + tonemap of the frames a slice completes runs on a second
+ * stream while the next slice draws.  This
  * generalises the reference's temporal-sample axis (one dispatch over T
  * parameter blocks, src/flame.cpp:275, shaders/templates/animate.tpl.glsl)
  * to frames of uneven size and xform count.  No reference counterpart. */
@@ -288,7 +291,9 @@ typedef struct rf_frame_desc {
 
 typedef struct rf_batch_config {
   int32_t block_waves;         /* 4 or 8: a work item is 64 * block_waves particles */
-  int32_t hist_mode;           /* RF_HIST_QUAD or RF_HIST_LDS */
+  int32_t hist_mode;           /* RF_HIST_QUAD, RF_HIST_LDS or RF_HIST_SORT (sort needs unit
+                                  opacities and palettes in [0, 1], else LDS or quad run;
+                                  its record capacities come from the last finished replay) */
   uint32_t iters;              /* binned iterations per particle */
   uint32_t fuse;               /* unbinned iterations per particle before those */
 } rf_batch_config;
@@ -408,8 +413,8 @@ int rf_set_flame_programs(rf_ctx* ctx, int32_t enable);
  * spent compiling its program in this process (0 when it came from the cache). */
 int rf_flame_program_info(const rf_flame* flame, double* compile_seconds);
 /* HIP source of the flame program of desc with one entry point for
- * (block_waves, hist_mode): hist_mode 1..5 for the fast engine, 16 + 1 or
- * 16 + 4 for the batched renderer's kernel.  Writes at most cap bytes
+ * (block_waves, hist_mode): hist_mode 1..5 for the fast engine, 16 + 1,
+ * 16 + 4 or 16 + 5 for the batched renderer's kernel.  Writes at most cap bytes
  * (NUL-terminated) and returns the full length + 1, or < 0 on error. */
 int rf_flame_program_source(const rf_flame_desc* desc, int32_t block_waves, int32_t hist_mode, char* buf,
                             size_t cap);

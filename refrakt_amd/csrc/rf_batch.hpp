@@ -6,6 +6,7 @@
 #endif
 
 #include "rf_device.hpp"
+#include "rf_sort.hpp"
 
 namespace rfk {
 
@@ -15,7 +16,8 @@ struct BatchFrame {
   uint32_t item_begin;    // first work item of the frame (frames are sorted by it)
   uint32_t n_items;
   float ss[6];            // screen affine
-  uint32_t pad[2];
+  uint32_t goff;          // first bin of the frame in the batch histogram (a multiple of 2^12)
+  uint32_t pad;
 };
 static_assert(sizeof(BatchFrame) == 56, "BatchFrame layout");
 
@@ -29,8 +31,15 @@ struct BatchArgs {
   unsigned long long* done_ts;    // per frame: wall clock when the last item was delivered
   uint64_t seed;
   uint32_t launch;
-  uint32_t n_frames, n_items;
+  uint32_t n_frames;
+  uint32_t item0, n_items;        // the launch's work items: item0 .. item0 + n_items - 1
   uint32_t fuse, n_iter;          // unbinned and binned iterations per particle
+  // RF_HIST_SORT: a launch (slice) stages records for the bins of its span,
+  // numbered from span_base in the batch histogram; span_bins points there
+  // (records beyond a bucket's capacity are added straight to it).
+  uint32_t span_base;
+  float* span_bins;
+  SortArgs sort;
 };
 
 #ifndef __HIPCC_RTC__

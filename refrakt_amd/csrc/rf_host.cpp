@@ -1049,9 +1049,10 @@ int rf_flame_program_info(const rf_flame* f, double* compile_seconds) {
 static int program_request(const rf_flame_desc* d, int32_t waves, int32_t hm, Block& b, std::string& err) {
   if (!d) { err = "null description"; return RF_ERR_ARG; }
   if (waves != 0 && waves != 4 && waves != 8) { err = "block_waves must be 0, 4 or 8"; return RF_ERR_ARG; }
-  const bool batch = hm == rfj::kBatchKernel + RF_HIST_QUAD || hm == rfj::kBatchKernel + RF_HIST_LDS;
+  const bool batch = hm == rfj::kBatchKernel + RF_HIST_QUAD || hm == rfj::kBatchKernel + RF_HIST_LDS ||
+                     hm == rfj::kBatchKernel + RF_HIST_SORT;
   if (!batch && (hm < RF_HIST_QUAD || hm > RF_HIST_SORT)) {
-    err = "hist_mode must be 1..5, or 16 + 1 or 16 + 4 for the batched renderer";
+    err = "hist_mode must be 1..5, or 16 + 1, 16 + 4 or 16 + 5 for the batched renderer";
     return RF_ERR_ARG;
   }
   if (!build_block(*d, b, err)) return RF_ERR_ARG;

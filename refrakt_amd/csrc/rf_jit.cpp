@@ -125,6 +125,8 @@ std::string program_source(const rfd::Block& b, const std::vector<std::pair<int,
     o << "extern \"C\" __global__ __launch_bounds__(" << 64 * W << ") ";
     if (mode < kBatchKernel)
       o << "__attribute__((amdgpu_waves_per_eu(rfk::detail::fast_waves_per_eu<" << mode << ">()))) ";
+    else if (mode == kBatchKernel + 5)
+      o << "__attribute__((amdgpu_waves_per_eu(6))) ";
     o << "void " << entry_name(W, mode);
     if (mode >= kBatchKernel) {
       o << "(rfk::BatchArgs g, const rfd::Block* __restrict__ blocks,\n"
@@ -265,7 +267,7 @@ hipError_t launch_batch(const Program* p, const rfk::BatchArgs& a0, int waves, i
   const std::pair<int, int> k{waves, kBatchKernel + hist_mode};
   const auto it = p->fns.find(k);
   if (it == p->fns.end()) return hipErrorNotFound;
-  uint32_t grid = std::min<uint32_t>(a0.n_items, (uint32_t)p->resident.at(k));
+  uint32_t grid = hist_mode == 5 ? a0.n_items : std::min<uint32_t>(a0.n_items, (uint32_t)p->resident.at(k));
   if (max_grid) grid = std::min(grid, max_grid);
   if (grid == 0) return hipSuccess;
   rfk::BatchArgs a = a0;

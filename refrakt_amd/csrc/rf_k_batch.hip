@@ -14,8 +14,8 @@ namespace {
 using namespace detail;
 
 template <int WAVES, int HM, int LOGK>
-__global__ __launch_bounds__(64 * WAVES) void k_batch(BatchArgs g, const Block* __restrict__ blocks,
-                                                      const BatchFrame* __restrict__ frames) {
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(HM == 5 ? 6 : 1)))
+void k_batch(BatchArgs g, const Block* __restrict__ blocks, const BatchFrame* __restrict__ frames) {
   k_batch_body<WAVES, HM, LOGK, InterpProg>(g, blocks, frames);
 }
 
@@ -61,7 +61,9 @@ static hipError_t launch_batch_k(const BatchArgs& a, uint32_t max_grid, hipStrea
   int resident = 0;
   const hipError_t e = resident_k<W, HM, LOGK>(&resident);
   if (e != hipSuccess) return e;
-  uint32_t grid = std::min<uint32_t>(a.n_items, (uint32_t)resident);
+  // LDS-table and direct modes: one workgroup per resident slot; sort
+  // staging: one per item, like the fast engine's sort kernel
+  uint32_t grid = HM == 5 ? a.n_items : std::min<uint32_t>(a.n_items, (uint32_t)resident);
   if (max_grid) grid = std::min(grid, max_grid);
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((k_batch<W, HM, LOGK>), dim3(grid), dim3(64 * W), 0, s, a, a.blocks, a.frames);
@@ -70,15 +72,21 @@ static hipError_t launch_batch_k(const BatchArgs& a, uint32_t max_grid, hipStrea
 
 hipError_t batch_prepare(int waves, int hist_mode) {
   int r = 0;
-  const bool lds = hist_mode == 4;
-  if (waves == 8) return lds ? resident_k<8, 4, 11>(&r) : resident_k<8, 1, 0>(&r);
-  return lds ? resident_k<4, 4, 10>(&r) : resident_k<4, 1, 0>(&r);
+  if (waves == 8) {
+    if (hist_mode == 4) return resident_k<8, 4, 11>(&r);
+    return hist_mode == 5 ? resident_k<8, 5, 0>(&r) : resident_k<8, 1, 0>(&r);
+  }
+  if (hist_mode == 4) return resident_k<4, 4, 10>(&r);
+  return hist_mode == 5 ? resident_k<4, 5, 0>(&r) : resident_k<4, 1, 0>(&r);
 }
 
 hipError_t launch_batch(const BatchArgs& a, int waves, int hist_mode, uint32_t max_grid, hipStream_t s) {
-  const bool lds = hist_mode == 4;
-  if (waves == 8) return lds ? launch_batch_k<8, 4, 11>(a, max_grid, s) : launch_batch_k<8, 1, 0>(a, max_grid, s);
-  return lds ? launch_batch_k<4, 4, 10>(a, max_grid, s) : launch_batch_k<4, 1, 0>(a, max_grid, s);
+  if (waves == 8) {
+    if (hist_mode == 4) return launch_batch_k<8, 4, 11>(a, max_grid, s);
+    return hist_mode == 5 ? launch_batch_k<8, 5, 0>(a, max_grid, s) : launch_batch_k<8, 1, 0>(a, max_grid, s);
+  }
+  if (hist_mode == 4) return launch_batch_k<4, 4, 10>(a, max_grid, s);
+  return hist_mode == 5 ? launch_batch_k<4, 5, 0>(a, max_grid, s) : launch_batch_k<4, 1, 0>(a, max_grid, s);
 }
 
 hipError_t launch_batch_init(const BatchArgs& a, float4* hist, uint64_t n_bins, const uint32_t* items,

@@ -8,7 +8,7 @@
 #endif
 
 #include "rf_batch.hpp"
-#include "rf_kern_common.hpp"
+#include "rf_k_fast_body.hpp"
 
 namespace rfk {
 namespace detail {
@@ -28,20 +28,32 @@ __device__ __forceinline__ CFrame* frame_constant(const BatchFrame* p) { return 
 // results do not depend on the grid.
 //
 // HM 4 keeps an LDS bin table per workgroup for the frame it is working on
-// and flushes it when the frame changes and at the end.  `remaining[f]`
-// counts the frame's items whose samples are not yet in its histogram; the
-// workgroup that delivers the last one stamps done_ts[f].
+// and flushes it when the frame changes and at the end.  HM 5 stages the
+// samples of an item in LDS and flushes them as bucket records of the
+// launch's span (the fast engine's sort staging, sort_flush); the record
+// passes that follow the launch add them to the histogram.  `remaining[f]`
+// counts the frame's items whose samples have left the chaos game (are in
+// its histogram, or in records); the workgroup that delivers the last one
+// stamps done_ts[f].
 template <int WAVES, int HM, int LOGK, class Prog>
 __device__ __forceinline__ void k_batch_body(const BatchArgs& g, const Block* __restrict__ blocks,
                                              const BatchFrame* __restrict__ frames) {
   constexpr int N = 64 * WAVES;
   __shared__ float s_x[2][N], s_y[2][N], s_c[2][N];
-  __shared__ float4 s_pal[256];
+  __shared__ float4 s_pal[HM == 5 ? 1 : 256];
   __shared__ uint16_t s_perm[NPERM * N];
   __shared__ LdsBins<LOGK> s_bins;
+  __shared__ SortStage<N, HM == 5> s_sort;
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < (uint32_t)(NPERM * N); i += N) s_perm[i] = g.perms[i];
   if constexpr (HM == 4) lds_bins_clear(s_bins);
+  if constexpr (HM == 5) {
+    for (uint32_t b = tid; b < g.sort.n_buckets; b += N) {
+      s_sort.boff[b] = g.sort.boff[b];
+      s_sort.bcap[b] = g.sort.bcap[b];
+      s_sort.cnt[b] = 0u;
+    }
+  }
 
   int cur = -1;              // frame whose palette (and LDS table) is loaded
   uint32_t pend = 0;         // items of `cur` not yet delivered
@@ -71,7 +83,8 @@ __device__ __forceinline__ void k_batch_body(const BatchArgs& g, const Block* __
     pend = 0;
   };
 
-  for (uint32_t item = blockIdx.x; item < g.n_items; item += gridDim.x) {
+  const uint32_t item_end = g.item0 + g.n_items;
+  for (uint32_t item = g.item0 + blockIdx.x; item < item_end; item += gridDim.x) {
     // frame of the item (frames are sorted by item_begin; the search is uniform)
     uint32_t f = 0;
     while (f + 1 < g.n_frames && frame_constant(frames + f + 1)->item_begin <= item) ++f;
@@ -79,7 +92,8 @@ __device__ __forceinline__ void k_batch_body(const BatchArgs& g, const Block* __
     if ((int)f != cur) {
       if (cur >= 0) deliver(cur);
       __syncthreads();
-      for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palettes[(size_t)f * 256 + i];
+      if constexpr (HM != 5)
+        for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palettes[(size_t)f * 256 + i];
       if constexpr (HM == 4) lds_bins_palette(s_bins, g.palettes + (size_t)f * 256);
       __syncthreads();
       cur = (int)f;
@@ -111,8 +125,15 @@ __device__ __forceinline__ void k_batch_body(const BatchArgs& g, const Block* __
       x = o.x; y = o.y; c = o.c;
       if (it >= g.fuse) {
         const Located L = locate<Prog>(blk, o, lrng, ss, W, H);
-        if constexpr (HM == 4) bin_lds(s_bins, bins, L.bidx, L.valid, L.pidx, L.w, s_pal, hits);
-        else bin_sample<HM>(bins, L.bidx, L.valid, L.pidx, L.w, s_pal);
+        if constexpr (HM == 4) {
+          bin_lds(s_bins, bins, L.bidx, L.valid, L.pidx, L.w, s_pal, hits);
+        } else if constexpr (HM == 5) {
+          const uint32_t slot = ((it - g.fuse) % sort_stage_iters<N>()) * N + tid;
+          s_sort.bin[slot] = L.valid ? F->goff + L.bidx - g.span_base : SORT_NONE;
+          s_sort.pid[slot] = (uint8_t)L.pidx;
+        } else {
+          bin_sample<HM>(bins, L.bidx, L.valid, L.pidx, L.w, s_pal);
+        }
         drawn += (unsigned long long)__popcll(__ballot(L.valid));
       }
       const uint32_t row = grng.next_a() & (uint32_t)(NPERM - 1);
@@ -121,6 +142,12 @@ __device__ __forceinline__ void k_batch_body(const BatchArgs& g, const Block* __
       s_x[b][dst] = x; s_y[b][dst] = y; s_c[b][dst] = c;
       lds_barrier();
       x = s_x[b][tid]; y = s_y[b][tid]; c = s_c[b][tid];
+      if constexpr (HM == 5) {
+        // the exchange barrier above also orders the staging writes before the flush
+        constexpr uint32_t SI = sort_stage_iters<N>();
+        if (it >= g.fuse && (((it - g.fuse) % SI) == SI - 1 || it + 1 == total))
+          sort_flush(s_sort, ((it - g.fuse) % SI) + 1, g.sort, g.span_bins, g.palettes + (size_t)f * 256);
+      }
     }
     lds_barrier();
     ++pend;

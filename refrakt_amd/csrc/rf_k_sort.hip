@@ -111,7 +111,9 @@ __global__ __launch_bounds__(RB_THREADS) void k_sort_rebucket(SortArgs a, float*
         pidx = r & 255u;
       }
     }
-    bin_quad(bins, bin, ovf, pidx, 1.0f, palette);  // overflow is rare: palette from global memory
+    // overflow is rare: palette from global memory (the tile's frame's, batched)
+    const float4* pal = a.tile_pal ? palette + a.tile_pal[bin >> SORT_TILE_SHIFT] : palette;
+    bin_quad(bins, bin, ovf, pidx, 1.0f, pal);
   }
 }
 
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(SortArgs a, float4*
   }
   if (tid < 256u) {
     // palette values lie in [0, 1]: each field gets at most `one` per record
-    const float4 c = palette[tid];
+    const float4 c = (a.tile_pal ? palette + a.tile_pal[t] : palette)[tid];
     const uint64_t r = (uint32_t)(c.x * one + 0.5f), g = (uint32_t)(c.y * one + 0.5f),
                    bl = (uint32_t)(c.z * one + 0.5f);
     s_t.pal[tid].rg = r | (g << 32);

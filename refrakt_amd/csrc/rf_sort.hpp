@@ -45,6 +45,10 @@ struct SortArgs {
   uint32_t n_buckets;
   uint32_t n_tiles;        // n_buckets * SORT_TILES_PER_BUCKET
   uint32_t n_chunks, n_segs;
+  // Batched rendering: per tile, the offset (in entries) of its frame's
+  // palette in the palette array the record passes get; null when one
+  // palette serves every tile.
+  const uint32_t* tile_pal;
 };
 
 #ifndef __HIPCC_RTC__

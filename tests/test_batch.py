@@ -113,7 +113,7 @@ def test_batch_autotune_picks_a_candidate(rf, gpu_available):
     ctx = rf.Context(1 << 12, 1, 16, seed=8)
     b = ctx.batch(_frames(rf))
     cfg, ms = b.tune(0.5)
-    assert cfg["block_waves"] in (4, 8) and cfg["hist_mode"] in (1, 4) and cfg["iters"] in (64, 256, 1024)
+    assert cfg["block_waves"] in (4, 8) and cfg["hist_mode"] in (1, 4, 5) and cfg["iters"] in (64, 256, 1024)
     assert ms > 0
     b.run()
     assert b.frame_stats()["drawn"].min() > 0
@@ -194,3 +194,69 @@ def test_cfg4_workload_against_fp64_oracle(rf, gpu_available):
     b.close()
     ctx.close()
     torch.cuda.synchronize()
+
+
+def _sort_vs_lds(rf, b, frames):
+    """Replays b in LDS mode, then in sort mode (capacities from that replay):
+    the trajectories are the same, so every frame's hit counts agree exactly;
+    colours within the sort path's fixed point (2^-16 per sample) plus the
+    float adds of n partial sums into a bin (n 2^-24 |sum|)."""
+    b.set_config(8, rf.RF_HIST_LDS, 64, 20)
+    b.run()
+    ref = [b.histogram(i) for i in range(len(frames))]
+    d_ref = b.frame_stats()["drawn"].copy()
+    b.set_config(8, rf.RF_HIST_SORT, 64, 20)
+    b.run()
+    d = b.frame_stats()["drawn"]
+    assert np.array_equal(d, d_ref)
+    for i in range(len(frames)):
+        h = b.histogram(i)
+        assert np.array_equal(h[..., 3], ref[i][..., 3]), i
+        n = ref[i][..., 3:4]
+        tol = 1e-6 + n * 2.0 ** -16 + n * 2.0 ** -24 * np.abs(ref[i][..., :3])
+        assert (np.abs(h[..., :3] - ref[i][..., :3]) <= tol).all(), i
+        assert int(h[..., 3].astype(np.float64).sum()) == int(d[i])
+
+
+def test_batch_sort_path_matches_lds(rf, gpu_available):
+    ctx = rf.Context(1 << 12, 1, 16, seed=5)
+    frames = _frames(rf)
+    b = ctx.batch(frames)
+    print("batch sort vs LDS", flush=True)
+    _sort_vs_lds(rf, b, frames)
+    b.close()
+
+
+def test_batch_sort_path_without_estimates(rf, gpu_available):
+    """A batch whose first graph uses the sort path has no replay to size its
+    record buffers from: uniform capacities overflow into quad atomics, and
+    the counts still equal the LDS path's."""
+    ctx = rf.Context(1 << 12, 1, 16, seed=5)
+    frames = _frames(rf)
+    b = ctx.batch(frames)
+    b.set_config(8, rf.RF_HIST_SORT, 64, 20)   # captured before any replay
+    b.run()
+    got = [b.histogram(i)[..., 3].copy() for i in range(len(frames))]
+    b.set_config(8, rf.RF_HIST_LDS, 64, 20)    # same batch: same random streams
+    b.run()
+    for i in range(len(frames)):
+        assert np.array_equal(got[i], b.histogram(i)[..., 3]), i
+    b.close()
+
+
+def test_batch_slices_finish_frames_early(rf, gpu_available):
+    """Frames of about 2^28 samples each: the replay runs in several slices
+    (at most 2^29 samples), and the first frames' images are done well before
+    the last's, because each slice's finished frames are density-estimated
+    while the next slice draws.  Sort and LDS modes agree on every frame's
+    counts across slices."""
+    ctx = rf.Context(1 << 12, 1, 16, seed=9)
+    frames = [(_flame(rf, 40 + i, 4, 512, 512), 512, 512, 260_000_000) for i in range(6)]
+    b = ctx.batch(frames)
+    print("batch slices", flush=True)
+    _sort_vs_lds(rf, b, frames)
+    st = b.frame_stats()
+    lat = st["image_done_ms"]
+    assert (np.diff(lat) >= 0).all()
+    assert lat[0] < 0.6 * lat[-1], lat
+    b.close()

@@ -150,3 +150,8 @@ def test_batch_program_compiles(rf):
     assert ("TERM_BEGIN_LOAD>(" in src.split("struct BProg")[1]) == (big.final_xform is not None)
     code, log, sec = rf.program_build(big, 8, 16 + 4, "gfx950")
     assert len(code) > 4096, log
+    # and with sort staging (mode 16 + 5)
+    src = rf.program_source(big, 8, 16 + 5)
+    assert "rf_batch_w8_h5" in src and "k_batch_body<8, 5, 0, rfprog::BProg>" in src
+    code, log, sec = rf.program_build(big, 8, 16 + 5, "gfx950")
+    assert len(code) > 4096, log
