@@ -8,6 +8,7 @@
 //   out: 32 -> 16, and the result is input[:4] + out[:4]
 // The input holds the 4 channels of log10(1 + v * W * H / n) of a histogram
 // (zero-padded to 16 channels); 1.18 M parameters.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -54,6 +55,7 @@ struct rf_denoiser {
   int N = 0, H = 0, W = 0;
   DevBuf e0a, cat0, p0, e1a, cat1, p1, e2a, cat2, p2, b, d2, d1, d0a, d0b, o;
   DevBuf g_e0a, g_cat0, g_p0, g_e1a, g_cat1, g_p1, g_e2a, g_cat2, g_p2, g_b, g_d2, g_d1, g_d0a, g_d0b, g_o;
+  DevBuf wg_work;  // per-split weight-gradient sums of one layer (fp32)
 };
 
 namespace {
@@ -73,6 +75,12 @@ int ensure_shape(rf_denoiser* d, int N, int H, int W) {
       {&d->g_d0a, P0 * 32}, {&d->g_d0b, P0 * 32}, {&d->g_o, P0 * 16},
   };
   for (const A& a : all) HIPCHK(ctx, a.b->alloc(a.n * sizeof(bf16)));
+  // layer l runs at 1/2^level of the image height
+  const int level[kLayers] = {0, 0, 1, 1, 2, 2, 3, 2, 1, 0, 0, 0};
+  size_t work = 0;
+  for (int l = 0; l < kLayers; ++l)
+    work = std::max(work, rfn::conv3x3_wgrad_work(N, H >> level[l], W >> level[l], d->L[l].cin, d->L[l].cout));
+  HIPCHK(ctx, d->wg_work.alloc(work * sizeof(float)));
   d->N = N; d->H = H; d->W = W;
   return RF_OK;
 }
@@ -165,7 +173,8 @@ int backward(rf_denoiser* d, const bf16* x, int N, int H, int W) {
   float* g = (float*)d->g.p;
   HIPCHK(ctx, rfn::zero_f32(g, d->n_params, s));
   auto wgrad = [&](int l, View xin, View dy, int h, int w) {
-    return rfn::conv3x3_wgrad(xin, dy, g + d->L[l].w_off, g + d->L[l].b_off, N, h, w, d->L[l].cin, d->L[l].cout, s);
+    return rfn::conv3x3_wgrad(xin, dy, g + d->L[l].w_off, g + d->L[l].b_off, N, h, w, d->L[l].cin, d->L[l].cout,
+                              (float*)d->wg_work.p, d->wg_work.bytes / sizeof(float), s);
   };
   auto dgrad = [&](int l, View dy, View dx, int h, int w) {
     return rfn::conv3x3(dy, wt_of(d, l), nullptr, dx, N, h, w, d->L[l].cout, d->L[l].cin, 0, s);
@@ -379,8 +388,12 @@ int rf_nn_conv3x3_wgrad(rf_ctx* ctx, const void* x, int32_t x_cs, int32_t x_co, 
                         int32_t dy_co, float* dw, float* db, int32_t N, int32_t H, int32_t W, int32_t Cin,
                         int32_t Cout) {
   if (!ctx || !x || !dy || !dw) return RF_ERR_ARG;
+  if (N <= 0 || H <= 0 || W <= 0 || Cin <= 0 || Cout <= 0 || Cin % 16 || Cout % 16) return RF_ERR_ARG;
+  DevBuf work;
+  const size_t n_work = rfn::conv3x3_wgrad_work(N, H, W, Cin, Cout);
+  HIPCHK(ctx, work.alloc(n_work * sizeof(float)));
   HIPCHK(ctx, rfn::conv3x3_wgrad(View{(bf16*)x, x_cs, x_co}, View{(bf16*)dy, dy_cs, dy_co}, dw, db, N, H, W, Cin,
-                                 Cout, ctx->stream));
+                                 Cout, (float*)work.p, n_work, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return RF_OK;
 }

@@ -393,66 +393,274 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RW == 2 ? 3
   }
 }
 
-// Weight gradient on the VALU: a workgroup owns a 16 (out) x 16 (in) channel
-// block, one thread per (k, c) pair with the 9 taps in registers, and walks
-// image rows; each 64-pixel segment of dy and the three input rows around it
-// are staged in LDS as fp32.  Partial sums go to dw/db with atomics.
-constexpr int WG_SEG = 64;
 
-__global__ __launch_bounds__(256) void k_wgrad(View x, View dy, float* __restrict__ dw, float* __restrict__ db,
-                                               int N, int H, int W, int Cin, int Cout) {
-  __shared__ float s_dy[WG_SEG][16];
-  __shared__ float s_x[3][WG_SEG + 2][17];
+// Weight gradient on MFMA: an implicit GEMM per tap (r, s) with M = output
+// channels, N = input channels, K = pixels:
+//   dw[r][s][k][c] = sum_p dy[p][k] x[p + (r - 1, s - 1)][c].
+// A workgroup owns a TCO x TCI channel tile, all nine taps, and a run of
+// "units" (one 64-pixel strip of one dy row; units are strip-major, so a
+// run walks down the rows of a strip).  The three input rows around the
+// current dy row live in a ring of LDS slots (image row yy in slot yy % 3):
+// moving one row down loads one new input row, a new image or strip two.
+// Staging is transposed to channel-major ([channel][pixel]) so a lane's eight
+// consecutive pixels are one 16-byte fragment read, and each input row is
+// stored at the three column shifts s = 0, 1, 2 so every tap reads aligned
+// fragments.  The next unit's rows are loaded into registers while the
+// current unit's MFMAs run (v_mfma_f32_16x16x32_bf16: lane l holds
+// A[l % 16][8 (l / 16) + e], B[8 (l / 16) + e][l % 16], D[4 (l / 16) + e][l % 16]).
+// Waves own SPW 16-channel output blocks x one 16-channel input block; when
+// the tile has fewer such groups than waves, WK = 2 waves split each strip's
+// two 32-pixel K steps and add in LDS at the end.  The bias gradient is the
+// product with an all-ones B.  Each run's sums go to its own slice of `part`
+// (no atomics); k_wgrad_reduce adds the slices to dw and db in a fixed order.
+constexpr int WG_SEG = 64, WG_XS = WG_SEG + 8;  // strip width, LDS row stride (bf16)
+
+template <int TCO, int TCI, int SPW>
+struct WgradCfg {
+  static constexpr int GROUPS = (TCO / (16 * SPW)) * (TCI / 16);  // wave groups of the tile
+  static constexpr int WK = GROUPS >= 4 ? 1 : 2;                  // waves per group (K split)
+  static_assert(GROUPS * WK <= 4 && (GROUPS == 4 || WK == 2), "four waves cover the tile");
+  static constexpr int PQO = TCO / 8, PQI = TCI / 8;               // 16-byte pieces per pixel
+  static constexpr int DY_Q = WG_SEG * PQO, XROW_Q = (WG_SEG + 2) * PQI;
+  static constexpr int NPER = (DY_Q + 3 * XROW_Q + 255) / 256;    // pieces per thread, at most 3 rows
+  static constexpr int LDS_X = 3 * 3 * TCI * WG_XS, LDS_DY = TCO * WG_XS;  // bf16 elements
+  static constexpr int RED = 10 * SPW * 4 * 64;                   // floats of the K-split reduction
+  static_assert(WK == 1 || GROUPS * RED * 4 <= LDS_X * 2, "the reduction fits the input ring");
+};
+
+// What one stage loads: dy row y of image n in strip xs and the nr (at most
+// three) consecutive input rows y0 .. y0 + nr - 1 of the same image and strip.
+struct WgradStage {
+  int n, y, xs, y0, nr;
+};
+
+__device__ __forceinline__ WgradStage wgrad_stage(int u, int H, int rows, bool fresh) {
+  WgradStage st;
+  const int strip = u / rows, row = u % rows;
+  st.n = row / H;
+  st.y = row % H;
+  st.xs = strip * WG_SEG;
+  // a run's first unit, a new image or a new strip: rows y - 1 .. y + 1;
+  // otherwise only y + 1 is new
+  const int lo = (fresh || st.y == 0) ? st.y - 1 : st.y + 1;
+  st.y0 = max(lo, 0);
+  st.nr = max(0, min(st.y + 1, H - 1) - st.y0 + 1);
+  return st;
+}
+
+template <int TCO, int TCI, int SPW>
+__device__ __forceinline__ void wgrad_load(uint4* v, const WgradStage& st,
+                                           const View& x, const View& dy, int H, int W, int ci0, int co0) {
+  using C = WgradCfg<TCO, TCI, SPW>;
   const int tid = threadIdx.x;
-  const int kl = tid >> 4, cl = tid & 15;
-  const int nkb = Cout / 16;
-  const int kb = blockIdx.y % nkb, cb = blockIdx.y / nkb;
-  const int k = kb * 16 + kl, c = cb * 16 + cl;
-  float acc[9];
+  const int total = C::DY_Q + st.nr * C::XROW_Q;
 #pragma unroll
-  for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
-  float accb = 0.0f;
-  const int rows = N * H;
-  for (int row = blockIdx.x; row < rows; row += gridDim.x) {
-    const int n = row / H, y = row % H;
-    for (int xs = 0; xs < W; xs += WG_SEG) {
-      __syncthreads();
-      for (int q = tid; q < WG_SEG * 16; q += 256) {
-        const int px = q >> 4, ch = q & 15;
-        const int xx = xs + px;
-        s_dy[px][ch] = xx < W ? bf2f(dy.p[((size_t)(n * H + y) * W + xx) * dy.cs + dy.co + kb * 16 + ch]) : 0.0f;
+  for (int k = 0; k < C::NPER; ++k) {
+    const int q = tid + 256 * k;
+    uint4 r = make_uint4(0u, 0u, 0u, 0u);
+    if (q < C::DY_Q) {
+      const int p = q / C::PQO, pc = q % C::PQO;
+      if (st.xs + p < W)
+        r = *reinterpret_cast<const uint4*>(dy.p + ((size_t)(st.n * H + st.y) * W + st.xs + p) * dy.cs + dy.co +
+                                            co0 + 8 * pc);
+    } else if (q < total) {
+      const int qq = q - C::DY_Q;
+      const int ri = qq / C::XROW_Q, rem = qq % C::XROW_Q;
+      const int p = rem / C::PQI, pc = rem % C::PQI;
+      const int gx = st.xs + p - 1;
+      if (gx >= 0 && gx < W)
+        r = *reinterpret_cast<const uint4*>(x.p + ((size_t)(st.n * H + st.y0 + ri) * W + gx) * x.cs + x.co + ci0 +
+                                            8 * pc);
+    }
+    v[k] = r;
+  }
+}
+
+template <int TCO, int TCI, int SPW>
+__device__ __forceinline__ void wgrad_store(const uint4* v, const WgradStage& st,
+                                            bf16* s_x, bf16* s_dy) {
+  using C = WgradCfg<TCO, TCI, SPW>;
+  const int tid = threadIdx.x;
+  const int total = C::DY_Q + st.nr * C::XROW_Q;
+#pragma unroll
+  for (int k = 0; k < C::NPER; ++k) {
+    const int q = tid + 256 * k;
+    const uint32_t w4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+    if (q < C::DY_Q) {
+      const int p = q / C::PQO, pc = q % C::PQO;
+      bf16* d = s_dy + (8 * pc) * WG_XS + p;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        d[(2 * e) * WG_XS] = (bf16)(w4[e] & 0xffffu);
+        d[(2 * e + 1) * WG_XS] = (bf16)(w4[e] >> 16);
       }
-      for (int q = tid; q < 3 * (WG_SEG + 2) * 16; q += 256) {
-        const int ch = q & 15, rest = q >> 4;
-        const int px = rest % (WG_SEG + 2), r = rest / (WG_SEG + 2);
-        const int yy = y + r - 1, xx = xs + px - 1;
-        float v = 0.0f;
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-          v = bf2f(x.p[((size_t)(n * H + yy) * W + xx) * x.cs + x.co + cb * 16 + ch]);
-        s_x[r][px][ch] = v;
-      }
-      __syncthreads();
-      float w0[3], w1[3];
+    } else if (q < total) {
+      const int qq = q - C::DY_Q;
+      const int ri = qq / C::XROW_Q, rem = qq % C::XROW_Q;
+      const int p = rem / C::PQI, pc = rem % C::PQI;
+      const int slot = (st.y0 + ri) % 3;
 #pragma unroll
-      for (int r = 0; r < 3; ++r) { w0[r] = s_x[r][0][cl]; w1[r] = s_x[r][1][cl]; }
-      for (int px = 0; px < WG_SEG; ++px) {
-        const float d = s_dy[px][kl];
-        accb += d;
+      for (int sh = 0; sh < 3; ++sh) {
+        // copy sh holds x[xs + j + sh - 1] at column j; staged pixel p is x[xs + p - 1]
+        const int j = p - sh;
+        if (j < 0 || j >= WG_SEG) continue;
+        bf16* d = s_x + ((slot * 3 + sh) * TCI + 8 * pc) * WG_XS + j;
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          const float w2 = s_x[r][px + 2][cl];
-          acc[r * 3 + 0] = fmaf(d, w0[r], acc[r * 3 + 0]);
-          acc[r * 3 + 1] = fmaf(d, w1[r], acc[r * 3 + 1]);
-          acc[r * 3 + 2] = fmaf(d, w2, acc[r * 3 + 2]);
-          w0[r] = w1[r];
-          w1[r] = w2;
+        for (int e = 0; e < 4; ++e) {
+          d[(2 * e) * WG_XS] = (bf16)(w4[e] & 0xffffu);
+          d[(2 * e + 1) * WG_XS] = (bf16)(w4[e] >> 16);
         }
       }
     }
   }
+}
+
+template <int TCO, int TCI, int SPW>
+__global__ __launch_bounds__(256) void k_wgrad_mfma(View x, View dy, float* __restrict__ part, int N, int H, int W,
+                                                    int Cin, int Cout, int runs, int with_bias) {
+  using C = WgradCfg<TCO, TCI, SPW>;
+  __shared__ __attribute__((aligned(16))) bf16 s_x[C::LDS_X];   // [slot][shift][ci][px]
+  __shared__ __attribute__((aligned(16))) bf16 s_dy[C::LDS_DY]; // [co][px]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i = lane & 15, kq = lane >> 4;
+  const int run = blockIdx.x, cb = blockIdx.y, kb = blockIdx.z;
+  const int ci0 = cb * TCI, co0 = kb * TCO;
+  const int rows = N * H, strips = (W + WG_SEG - 1) / WG_SEG, units = rows * strips;
+  const int u0 = (int)((long long)units * run / runs), u1 = (int)((long long)units * (run + 1) / runs);
+  // this wave's group: output blocks sco0 .. sco0 + SPW - 1, input block sci, K steps kw, kw + WK, ...
+  const int kw = wave % C::WK, grp = wave / C::WK;
+  const bool active = grp < C::GROUPS;
+  const int sci = grp % (TCI / 16), sco0 = (grp / (TCI / 16)) * SPW;
+  const bool bias = with_bias && cb == 0 && sci == 0 && active;
+  float4_t acc[9][SPW], accb[SPW];
 #pragma unroll
-  for (int q = 0; q < 9; ++q) atomicAdd(dw + ((size_t)q * Cout + k) * Cin + c, acc[q]);
-  if (db && cb == 0 && cl == 0) atomicAdd(db + k, accb);
+  for (int q = 0; q < 9; ++q)
+#pragma unroll
+    for (int t = 0; t < SPW; ++t) acc[q][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < SPW; ++t) accb[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const uint4 one4 = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);
+  const bf16x8_t ones = __builtin_bit_cast(bf16x8_t, one4);
+  uint4 v[C::NPER];
+  if (u0 < u1) {
+    WgradStage st = wgrad_stage(u0, H, rows, true);
+    wgrad_load<TCO, TCI, SPW>(v, st, x, dy, H, W, ci0, co0);
+    wgrad_store<TCO, TCI, SPW>(v, st, s_x, s_dy);
+    __syncthreads();
+    for (int u = u0; u < u1; ++u) {
+      const WgradStage cur = st;
+      const bool more = u + 1 < u1;
+      if (more) {
+        st = wgrad_stage(u + 1, H, rows, false);
+        wgrad_load<TCO, TCI, SPW>(v, st, x, dy, H, W, ci0, co0);
+      }
+      if (active) {
+#pragma unroll
+        for (int ks = kw; ks < WG_SEG / 32; ks += C::WK) {
+          const int px = 32 * ks + 8 * kq;
+          bf16x8_t a[SPW];
+#pragma unroll
+          for (int t = 0; t < SPW; ++t)
+            a[t] = *reinterpret_cast<const bf16x8_t*>(s_dy + (16 * (sco0 + t) + i) * WG_XS + px);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const int yy = cur.y + r - 1;
+            if (yy < 0 || yy >= H) continue;  // zero padding: no products
+            const int slot = yy % 3;
+#pragma unroll
+            for (int sh = 0; sh < 3; ++sh) {
+              const bf16x8_t b =
+                  *reinterpret_cast<const bf16x8_t*>(s_x + ((slot * 3 + sh) * TCI + 16 * sci + i) * WG_XS + px);
+#pragma unroll
+              for (int t = 0; t < SPW; ++t)
+                acc[r * 3 + sh][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b, acc[r * 3 + sh][t], 0, 0, 0);
+            }
+          }
+          if (bias) {
+#pragma unroll
+            for (int t = 0; t < SPW; ++t) accb[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], ones, accb[t], 0, 0, 0);
+          }
+        }
+      }
+      __syncthreads();  // the ring slot and dy buffer the next stage overwrites are read
+      if (more) {
+        wgrad_store<TCO, TCI, SPW>(v, st, s_x, s_dy);
+        __syncthreads();
+      }
+    }
+  }
+  // K-split groups: the second wave of each group adds its sums to the first's
+  if (C::WK == 2) {
+    float* s_red = reinterpret_cast<float*>(s_x);  // free: the last unit's reads ended at a barrier
+    if (active && kw == 1) {
+#pragma unroll
+      for (int q = 0; q < 10; ++q)
+#pragma unroll
+        for (int t = 0; t < SPW; ++t)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            s_red[((grp * 10 + q) * SPW + t) * 4 * 64 + e * 64 + lane] = q < 9 ? acc[q][t][e] : accb[t][e];
+    }
+    __syncthreads();
+    if (active && kw == 0) {
+#pragma unroll
+      for (int q = 0; q < 10; ++q)
+#pragma unroll
+        for (int t = 0; t < SPW; ++t)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float o = s_red[((grp * 10 + q) * SPW + t) * 4 * 64 + e * 64 + lane];
+            if (q < 9) acc[q][t][e] += o;
+            else accb[t][e] += o;
+          }
+    }
+  }
+  if (!active || kw != 0) return;
+  // part[run][q][co][ci]; the bias sums of every run follow all runs' weights
+  const size_t per = (size_t)9 * Cout * Cin;
+  float* const pr = part + (size_t)run * per;
+#pragma unroll
+  for (int q = 0; q < 9; ++q)
+#pragma unroll
+    for (int t = 0; t < SPW; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int co = co0 + 16 * (sco0 + t) + 4 * kq + e, ci = ci0 + 16 * sci + i;
+        pr[((size_t)q * Cout + co) * Cin + ci] = acc[q][t][e];
+      }
+  if (bias && i == 0) {
+    // every column of the all-ones product holds the row sums
+#pragma unroll
+    for (int t = 0; t < SPW; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        part[(size_t)runs * per + (size_t)run * Cout + co0 + 16 * (sco0 + t) + 4 * kq + e] = accb[t][e];
+  }
+}
+
+// dw[i] += sum over runs of part (bias entries after the weights): 64
+// outputs per workgroup, 16 lanes per output summing every 16th run, the 16
+// sums added in a fixed order (deterministic).
+__global__ __launch_bounds__(1024) void k_wgrad_reduce(const float* __restrict__ part, float* __restrict__ dw,
+                                                       float* __restrict__ db, int runs, int n_w, int n_b) {
+  __shared__ float s_sum[16][65];
+  const int o = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int idx = blockIdx.x * 64 + o;
+  float acc = 0.0f;
+  if (idx < n_w) {
+    for (int k = g; k < runs; k += 16) acc += part[(size_t)k * n_w + idx];
+  } else if (idx < n_w + n_b) {
+    for (int k = g; k < runs; k += 16) acc += part[(size_t)runs * n_w + (size_t)k * n_b + (idx - n_w)];
+  }
+  s_sum[g][o] = acc;
+  __syncthreads();
+  if (g == 0) {
+    float t = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += s_sum[k][o];
+    if (idx < n_w) dw[idx] += t;
+    else if (idx < n_w + n_b) db[idx - n_w] += t;
+  }
 }
 
 __global__ void k_flip(const bf16* __restrict__ w, bf16* __restrict__ wt, int Cin, int Cout) {
@@ -726,12 +934,56 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
   return e;
 }
 
-hipError_t conv3x3_wgrad(View x, View dy, float* dw, float* db, int N, int H, int W, int Cin, int Cout,
-                         hipStream_t s) {
-  if (Cin % 16 || Cout % 16) return hipErrorInvalidValue;
-  const int rows = N * H;
-  const dim3 grid((unsigned)std::min(rows, 256), (unsigned)((Cout / 16) * (Cin / 16)));
-  hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, s, x, dy, dw, db, N, H, W, Cin, Cout);
+namespace {
+// Channel tile of the weight gradient: TCO x TCI, SPW output blocks per wave.
+struct WgradTile {
+  int tco, tci;
+};
+WgradTile wgrad_tile(int Cin, int Cout) {
+  const int tci = Cin % 32 == 0 ? 32 : 16;
+  const int tco = (Cout % 64 == 0 && tci == 32) ? 64 : Cout % 32 == 0 ? 32 : 16;
+  return WgradTile{tco, tci};
+}
+// Runs (row-strip ranges with their own partial sums): about 2048
+// workgroups, at least 4 units per run, at most 2^23 partial sums.
+int wgrad_runs(int N, int H, int W, int Cin, int Cout) {
+  const WgradTile t = wgrad_tile(Cin, Cout);
+  const int blocks = (Cout / t.tco) * (Cin / t.tci);
+  const long long units = (long long)N * H * ((W + WG_SEG - 1) / WG_SEG);
+  const size_t per = (size_t)9 * Cout * Cin + Cout;
+  long long runs = std::min<long long>((2048 + blocks - 1) / blocks, std::max<long long>(1, units / 4));
+  runs = std::min<long long>(runs, (long long)std::max<size_t>(1, ((size_t)1 << 23) / per));
+  return (int)std::max<long long>(1, runs);
+}
+
+template <int TCO, int TCI, int SPW>
+void launch_wgrad(dim3 grid, hipStream_t s, View x, View dy, float* work, int N, int H, int W, int Cin, int Cout,
+                  int runs, int with_bias) {
+  hipLaunchKernelGGL((k_wgrad_mfma<TCO, TCI, SPW>), grid, dim3(256), 0, s, x, dy, work, N, H, W, Cin, Cout, runs,
+                     with_bias);
+}
+}  // namespace
+
+size_t conv3x3_wgrad_work(int N, int H, int W, int Cin, int Cout) {
+  return (size_t)wgrad_runs(N, H, W, Cin, Cout) * ((size_t)9 * Cout * Cin + Cout);
+}
+
+hipError_t conv3x3_wgrad(View x, View dy, float* dw, float* db, int N, int H, int W, int Cin, int Cout, float* work,
+                         size_t work_floats, hipStream_t s) {
+  if (Cin % 16 || Cout % 16 || N <= 0 || H <= 0 || W <= 0 || !work) return hipErrorInvalidValue;
+  if (work_floats < conv3x3_wgrad_work(N, H, W, Cin, Cout)) return hipErrorInvalidValue;
+  const WgradTile t = wgrad_tile(Cin, Cout);
+  const int runs = wgrad_runs(N, H, W, Cin, Cout);
+  const dim3 grid((unsigned)runs, (unsigned)(Cin / t.tci), (unsigned)(Cout / t.tco));
+  const int wb = db ? 1 : 0;
+  if (t.tco == 64) launch_wgrad<64, 32, 2>(grid, s, x, dy, work, N, H, W, Cin, Cout, runs, wb);
+  else if (t.tco == 32 && t.tci == 32) launch_wgrad<32, 32, 1>(grid, s, x, dy, work, N, H, W, Cin, Cout, runs, wb);
+  else if (t.tco == 32) launch_wgrad<32, 16, 1>(grid, s, x, dy, work, N, H, W, Cin, Cout, runs, wb);
+  else if (t.tci == 32) launch_wgrad<16, 32, 1>(grid, s, x, dy, work, N, H, W, Cin, Cout, runs, wb);
+  else launch_wgrad<16, 16, 1>(grid, s, x, dy, work, N, H, W, Cin, Cout, runs, wb);
+  const int n_w = 9 * Cout * Cin, n_b = db ? Cout : 0;
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((n_w + n_b + 63) / 64)), dim3(1024), 0, s, work, dw, db, runs,
+                     n_w, n_b);
   return hipGetLastError();
 }
 

@@ -41,9 +41,13 @@ struct ConvFuse {
 hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, int H, int W, int Cin, int Cout,
                    int relu, hipStream_t s, const ConvFuse& fuse = ConvFuse{});
 // Weight gradient: dw[r][s][k][c] += sum_p dy[p,k] * x[p + (r-1, s-1), c] and
-// db[k] += sum_p dy[p,k] (fp32, accumulated with atomics; zero them first).
-hipError_t conv3x3_wgrad(View x, View dy, float* dw, float* db, int N, int H, int W, int Cin, int Cout,
-                         hipStream_t s);
+// db[k] += sum_p dy[p,k] (bf16 products on the matrix cores, fp32 sums).  The
+// image rows are split over workgroups whose partial sums go to `work`
+// (conv3x3_wgrad_work(N, H, W, Cin, Cout) floats), then are added to dw/db in a
+// fixed order: deterministic, no atomics.
+size_t conv3x3_wgrad_work(int N, int H, int W, int Cin, int Cout);
+hipError_t conv3x3_wgrad(View x, View dy, float* dw, float* db, int N, int H, int W, int Cin, int Cout, float* work,
+                         size_t work_floats, hipStream_t s);
 // w_t[r][s][c][k] = w[2-r][2-s][k][c]: the weights of the input-gradient
 // convolution (dx = conv3x3(dy, w_t) with Cin' = Cout, Cout' = Cin).
 hipError_t flip_weights(const bf16* w, bf16* wt, int Cin, int Cout, hipStream_t s);

@@ -179,10 +179,34 @@ def time_inference(sizes, reps: int = 3, seed: int = 1):
     return out
 
 
+def time_train_step(batch: int = 8, crop: int = 256, reps: int = 10, seed: int = 1):
+    """Host wall time of one training step (forward, backward, Adam) on random
+    bf16 inputs and targets of batch x crop x crop, after two warm-up steps."""
+    import torch
+    ctx = Context(1 << 12, 1, 16)
+    dn = ctx.denoiser(seed)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    x = (torch.rand((batch, crop, crop, 16), device="cuda", generator=g) * 2).to(torch.bfloat16)
+    t = (torch.rand((batch, crop, crop, 4), device="cuda", generator=g) * 2).to(torch.bfloat16)
+    for _ in range(2):
+        dn.train_step(x.data_ptr(), t.data_ptr(), batch, crop, crop, 1e-4, want_loss=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dn.train_step(x.data_ptr(), t.data_ptr(), batch, crop, crop, 1e-4, want_loss=False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    out = {"event": "train_step", "batch": batch, "crop": crop, "ms": round(ms, 3)}
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--time", default=None,
                     help="time inference instead of training, e.g. 4320x7680,1024x1024 (height x width)")
+    ap.add_argument("--time-train", default=None,
+                    help="time training steps instead of training, e.g. 8x256 (batch x crop)")
     ap.add_argument("--flames", type=int, default=2000)
     ap.add_argument("--val-flames", type=int, default=64)
     ap.add_argument("--steps", type=int, default=3000)
@@ -194,6 +218,10 @@ def main():
     a = ap.parse_args()
     if a.time:
         time_inference([tuple(int(v) for v in s.split("x")) for s in a.time.split(",")])
+        return
+    if a.time_train:
+        b, c = (int(v) for v in a.time_train.split("x"))
+        time_train_step(b, c)
         return
     train(a.flames, a.steps, a.batch, a.crop, a.lr, a.seed, a.out, val_flames=a.val_flames)
 

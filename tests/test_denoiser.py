@@ -88,13 +88,17 @@ def test_conv3x3_matches_torch(rf, gpu_available, Cin, Cout, W, relu, cs_in, co_
     assert (rest == 0).all()  # channels outside the output view are untouched
 
 
-@pytest.mark.parametrize("Cin,Cout,W", [(16, 32, 70), (96, 32, 40), (64, 128, 24)])
-def test_conv3x3_gradients_match_torch(rf, gpu_available, Cin, Cout, W):
+# Channel tiles of the weight gradient: 32x16 (16, 32), 32x32 (96, 32),
+# 64x32 (64, 128), 16x32 (32, 16), 16x16 (48, 16); runs of rows that cross
+# 64-pixel strips (W 70, 130) and images; (384, 128, 16, 16): 32 rows, 8 runs.
+@pytest.mark.parametrize("Cin,Cout,W,H", [(16, 32, 70, 10), (96, 32, 40, 10), (64, 128, 24, 10),
+                                          (384, 128, 16, 16), (32, 16, 130, 6), (48, 16, 33, 3)])
+def test_conv3x3_gradients_match_torch(rf, gpu_available, Cin, Cout, W, H):
     import torch
     import torch.nn.functional as F
     ctx = rf.Context(1 << 12, 1, 16)
     g = torch.Generator().manual_seed(Cin + 7 * Cout)
-    N, H = 2, 10
+    N = 2
     x = _bf16(torch.randn(N, H, W, Cin, generator=g))
     dy = _bf16(torch.randn(N, H, W, Cout, generator=g))
     w = _bf16(0.1 * torch.randn(3, 3, Cout, Cin, generator=g))
