@@ -40,10 +40,13 @@ struct Term {            // 48 bytes
   uint32_t pad;
 };
 
-struct Xform {           // 80 bytes
+struct Xform {           // 88 bytes
   float aff[6];          // a b c d e f ; index = col*2 + row (c00 c01 c10 c11 c20 c21)
   float post[6];
   float color, color_speed, opacity;
+  // the colour blend's uniform factors, rounded as rf_mix rounds them:
+  // c_keep = 1 - color_speed, c_add = color * color_speed
+  float c_keep, c_add;
   uint32_t flags;
   uint32_t precalc;      // union of the regular terms' precalc needs
   uint32_t term_begin;   // first term in Block::terms
@@ -186,9 +189,9 @@ __device__ __forceinline__ float rf_div_rn(float x, float y) { return (float)((d
 // Palette index min(255, uint(ceil(c * 255))) (shaders/flame.glsl:84), with
 // negative and NaN colours mapped to 0 (undefined in GLSL).
 __host__ __device__ __forceinline__ uint32_t rf_pal_index(float c) {
-  float t = ceilf(c * 255.0f);
-  if (!(t > 0.0f)) return 0u;
-  if (t >= 255.0f) return 255u;
+  // ceil(c * 255) clamped to [0, 255], NaN to 0, without branches: fmaxf
+  // returns its non-NaN operand
+  const float t = fminf(fmaxf(ceilf(c * 255.0f), 0.0f), 255.0f);
   return (uint32_t)t;
 }
 
@@ -363,7 +366,9 @@ __device__ __forceinline__ XOut finish_xform(XP xf, bool post, float rx, float r
   float cin = first ? rng.randf() : c0;
   XOut o;
   o.x = rx; o.y = ry;
-  o.c = rf_mix(cin, xf->color, xf->color_speed);
+  // rf_mix(cin, color, color_speed) with its uniform factors precomputed
+  const float t0 = cin * xf->c_keep;
+  o.c = t0 + xf->c_add;
   o.w = xf->opacity;
   return o;
 }

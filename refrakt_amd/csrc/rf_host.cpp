@@ -155,6 +155,11 @@ bool build_block(const rf_flame_desc& d, Block& b, std::string& err) {
     X.color = x.color;
     X.color_speed = x.color_speed;
     X.opacity = x.opacity;
+    // rf_mix's factors in single precision (no contraction: the same
+    // roundings as on the device)
+    const float keep = 1.0f - x.color_speed, add = x.color * x.color_speed;
+    X.c_keep = keep;
+    X.c_add = add;
     if (x.n_vars < 0 || x.n_vars > RF_MAX_VARS_PER_XFORM) { err = "n_vars out of range"; return false; }
     // std::map<std::string,float> semantics: one entry per name (last wins),
     // iterated in name order (src/flame.hpp:26).

@@ -134,7 +134,7 @@ __device__ __forceinline__ void k_batch_body(const BatchArgs& g, const Block* __
         } else {
           bin_sample<HM>(bins, L.bidx, L.valid, L.pidx, L.w, s_pal);
         }
-        drawn += (unsigned long long)__popcll(__ballot(L.valid));
+        drawn += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(L.valid));
       }
       const uint32_t row = grng.next_a() & (uint32_t)(NPERM - 1);
       const uint32_t dst = s_perm[row * N + tid];

@@ -279,7 +279,7 @@ __device__ __forceinline__ void k_fast_body(const FastArgs& g, const Block* __re
         } else {
           bin_sample<HM>(g.bins, L.bidx, L.valid, L.pidx, L.w, s_pal);
         }
-        drawn += (unsigned long long)__popcll(__ballot(L.valid));
+        drawn += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(L.valid));
       }
       const uint32_t row = grng.next_a() & (uint32_t)(NPERM - 1);
       const uint32_t dst = s_perm[row * N + tid];

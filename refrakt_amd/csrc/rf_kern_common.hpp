@@ -315,12 +315,11 @@ __device__ __forceinline__ Located locate(BP blk, const XOut& o, Rng& rng, const
   const float sx = fmaf(ss[0], rx, fmaf(ss[2], ry, ss[4]));
   const float sy = fmaf(ss[1], rx, fmaf(ss[3], ry, ss[5]));
   Located L;
-  L.valid = (sx >= 0.0f) && (sx < (float)W) && (sy >= 0.0f) && (sy < (float)H) && (rw > 0.0f);
-  L.bidx = 0;
-  if (L.valid) {
-    const uint32_t cx = (uint32_t)floorf(sx), cy = (uint32_t)floorf(sy);
-    L.bidx = (H - cy - 1u) * W + cx;
-  }
+  // straight-line code: the bin is computed for every lane and kept only
+  // where valid (a conversion of an out-of-range value is harmless there)
+  L.valid = (sx >= 0.0f) & (sx < (float)W) & (sy >= 0.0f) & (sy < (float)H) & (rw > 0.0f);
+  const uint32_t cx = (uint32_t)floorf(sx), cy = (uint32_t)floorf(sy);
+  L.bidx = L.valid ? (H - cy - 1u) * W + cx : 0u;
   L.pidx = rf_pal_index(rc);
   L.w = rw;
   return L;
