@@ -380,6 +380,14 @@ int rf_rgba8_device(rf_ctx* ctx, const void* in_device, void* out_device, uint32
 /* Writes W*H RGBA8 pixels (first row first) as a PNG file, ≙ the screenshot
  * of src/main.cpp:590-593. */
 int rf_write_png(const char* path, const uint8_t* rgba8, uint32_t width, uint32_t height);
+/* Writes W*H RGBA float pixels (first row first) as an uncompressed
+ * single-part scanline OpenEXR file, channels stored as HALF (half != 0) or
+ * FLOAT.  No counterpart in the reference, which saves 8-bit PNG only
+ * (src/main.cpp:590-593): for the density-estimated or tonemapped float
+ * image without quantisation. */
+int rf_write_exr(const char* path, const float* rgba, uint32_t width, uint32_t height, int32_t half);
+/* IEEE binary32 -> binary16, round to nearest even (the EXR HALF encoding). */
+uint16_t rf_half_from_float(float f);
 
 /* ---------------------------------------------------- state inspection */
 /* Particle buffer (P x (x, y, color, 0)) after the last pass. */

@@ -164,6 +164,8 @@ _SIGS = {
     "rf_rgba8_from_f32": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
     "rf_rgba8_device": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "rf_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),
+    "rf_write_exr": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32, C.c_int32]),
+    "rf_half_from_float": (C.c_uint16, [C.c_float]),
     "rf_screen_affine": (C.c_int, [C.POINTER(FlameDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     "rf_set_flame_programs": (C.c_int, [_P, C.c_int32]),
     "rf_batch_program_info": (C.c_int, [_P, C.POINTER(C.c_double)]),
@@ -360,6 +362,17 @@ def write_png(path: str, rgba8: np.ndarray) -> None:
     rc = lib().rf_write_png(os.fsencode(path), rgba8.ctypes.data, rgba8.shape[1], rgba8.shape[0])
     if rc != RF_OK:
         raise RefraktError(f"rf_write_png failed ({rc}) for {path}")
+
+
+def write_exr(path: str, rgba: np.ndarray, half: bool = True) -> None:
+    """Writes an (H, W, 4) float RGBA image as an uncompressed OpenEXR file
+    (first row at the top; channels HALF, or FLOAT when half is False)."""
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    if rgba.ndim != 3 or rgba.shape[2] != 4:
+        raise ValueError("expected an (H, W, 4) float image")
+    rc = lib().rf_write_exr(os.fsencode(path), rgba.ctypes.data, rgba.shape[1], rgba.shape[0], 1 if half else 0)
+    if rc != RF_OK:
+        raise RefraktError(f"rf_write_exr failed ({rc}) for {path}")
 
 
 def param_block(flame: Flame) -> np.ndarray:

@@ -109,3 +109,79 @@ def test_png_round_trip_and_rgba8_conversion(rf, tmp_path):
     path = str(tmp_path / "x.png")
     rf.write_png(path, q)
     assert np.array_equal(_read_png(path), q)
+
+
+def _read_exr(path):
+    """Minimal reader of the files rf_write_exr writes (single-part scanline,
+    uncompressed): header attributes, offset table and (H, W, 4) RGBA pixels
+    as raw uint16 (HALF) or float32 (FLOAT) values."""
+    import struct
+    b = open(path, "rb").read()
+    assert b[:4] == bytes([0x76, 0x2F, 0x31, 0x01]) and b[4:8] == bytes([2, 0, 0, 0])
+    p, attrs = 8, {}
+    while b[p] != 0:
+        e = b.index(0, p); name = b[p:e].decode(); p = e + 1
+        e = b.index(0, p); typ = b[p:e].decode(); p = e + 1
+        (n,) = struct.unpack_from("<i", b, p); p += 4
+        attrs[name] = (typ, b[p:p + n]); p += n
+    p += 1
+    chans, q = [], attrs["channels"][1]
+    while q[0] != 0:
+        e = q.index(0); nm = q[:e].decode(); q = q[e + 1:]
+        pt, lin, xs, ys = struct.unpack_from("<iB3xii", q); q = q[16:]
+        chans.append((nm, pt, lin, xs, ys))
+    x0, y0, x1, y1 = struct.unpack("<4i", attrs["dataWindow"][1])
+    W, H = x1 - x0 + 1, y1 - y0 + 1
+    offs = struct.unpack_from(f"<{H}Q", b, p)
+    pt = chans[0][1]
+    dt = np.dtype("<u2") if pt == 1 else np.dtype("<f4")
+    img = np.zeros((H, W, 4), dt)
+    for y in range(H):
+        yy, n = struct.unpack_from("<ii", b, offs[y])
+        assert yy == y and n == W * 4 * dt.itemsize
+        line = np.frombuffer(b, dt, W * 4, offs[y] + 8).reshape(4, W)
+        for k, (nm, _, _, _, _) in enumerate(chans):
+            img[y, :, "RGBA".index(nm)] = line[k]
+    return attrs, chans, img
+
+
+def test_half_from_float_matches_ieee_round_to_nearest_even(rf):
+    import ctypes
+    L = rf.lib()
+    edge = np.array([0.0, -0.0, 1.0, -2.5, 65504.0, 65519.99, 65520.0, 1e6, 2.0 ** -14, 2.0 ** -24, 2.0 ** -25,
+                     np.nextafter(np.float32(2.0 ** -25), np.float32(1)), 3 * 2.0 ** -25, 1.5 * 2.0 ** -24,
+                     2.0 ** -14 - 2.0 ** -25, 1 + 2.0 ** -11, 1 + 3 * 2.0 ** -11, 1e-10, np.inf, -np.inf],
+                    np.float32)
+    rng = np.random.default_rng(9)
+    rnd = np.concatenate([edge, rng.standard_normal(20000).astype(np.float32) * np.float32(10.0) **
+                          rng.integers(-9, 6, 20000).astype(np.float32)]).astype(np.float32)
+    got = np.array([L.rf_half_from_float(ctypes.c_float(float(v))) for v in rnd], np.uint16)
+    with np.errstate(over="ignore"):  # values past 65504 become infinity, as they should
+        ref = rnd.astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, ref)
+    nan = L.rf_half_from_float(ctypes.c_float(float("nan")))
+    assert np.isnan(np.array([nan], np.uint16).view(np.float16)[0])
+
+
+@pytest.mark.parametrize("half", [True, False])
+def test_exr_round_trip(rf, tmp_path, half):
+    rng = np.random.default_rng(3)
+    img = (rng.uniform(-1, 4, size=(19, 31, 4)) * 10.0 ** rng.integers(-6, 5, size=(19, 31, 4))).astype(np.float32)
+    img[0, 0] = [np.inf, -np.inf, 0.0, 70000.0]
+    img[1, 2, 1] = np.nan
+    path = str(tmp_path / "x.exr")
+    rf.write_exr(path, img, half=half)
+    attrs, chans, got = _read_exr(path)
+    assert [c[0] for c in chans] == ["A", "B", "G", "R"]
+    assert all(c[1] == (1 if half else 2) and c[3] == 1 and c[4] == 1 for c in chans)
+    assert attrs["compression"] == ("compression", b"\x00")
+    assert attrs["lineOrder"] == ("lineOrder", b"\x00")
+    assert attrs["dataWindow"][1] == attrs["displayWindow"][1]
+    if half:
+        with np.errstate(over="ignore"):
+            ref = img.astype(np.float16).view(np.uint16)
+        nan = np.isnan(img)
+        assert np.array_equal(got[~nan], ref[~nan])
+        assert np.isnan(got.view(np.float16)[nan]).all()
+    else:
+        assert np.array_equal(got.view(np.uint32), img.view(np.uint32))
