@@ -267,6 +267,30 @@ def test_every_variation_vs_fp64_oracle(rf, orc, gpu_available):
     assert not failures, failures
 
 
+# Error budget of the fast engine's sine (the argument in revolutions, x / 2 pi
+# rounded to float, into the hardware v_sin_f32): the reduction's rounding
+# grows with |x|, so the stated bound per point is 1e-6 + 1.5e-7 |x| (GLSL
+# specifies sin only on [-pi, pi], to 2^-11).  Measured maxima: 4.4e-7 on
+# [-pi, pi], 7.8e-7 on [-10, 10], 6.8e-6 on [-100, 100], 8.7e-5 on
+# [-1000, 1000].
+@pytest.mark.parametrize("xmax", [np.pi, 10.0, 100.0, 1000.0])
+def test_sine_error_budget_per_range(rf, gpu_available, xmax):
+    ctx = _ctx(rf, 1 << 10, 1, rf.RF_ENGINE_FAST)
+    rng = np.random.default_rng(int(xmax))
+    n = 65536
+    pts = np.zeros((n, 3), np.float32)
+    pts[:, 0] = rng.uniform(-xmax, xmax, n).astype(np.float32)
+    pts[:, 1] = rng.uniform(-xmax, xmax, n).astype(np.float32)
+    rands = np.zeros((n, 16), np.float32)
+    xf = rf.Xform(affine=[1.0, 0.0, 0.0, 1.0, 0.0, 0.0], variations={"sinusoidal": 1.0}, color=0.0,
+                  color_speed=0.0, opacity=1.0)
+    g = ctx.eval_xform(xf, pts, rands)
+    x = pts[:, :2].astype(np.float64)
+    err = np.abs(g[:, :2].astype(np.float64) - np.sin(x))
+    bound = 1e-6 + 1.5e-7 * np.abs(x)
+    assert (err <= bound).all(), (xmax, float(err.max()), float((err / bound).max()))
+
+
 # ------------------------------------------------------------- DE, tonemap
 def _sparse_hist(rng, W, H, density=0.3):
     h = np.zeros((H, W, 4), np.float32)

@@ -381,6 +381,66 @@ def test_density_stamp_mass_closed_form(orc, r):
     assert np.abs(nz[:, 0] - cy).max() <= r and np.abs(nz[:, 1] - cx).max() <= r
 
 
+def _gl_point_splat(h, R, rmin, curve):
+    """The density pass restated from the OpenGL 4.6 rules, independently of
+    the oracle's closed form (float64): one point per bin at object position
+    pos - offset, offset = (hw, -hw), hw = 1/(4r + 2) (density_vert.glsl:27-60)
+    through ortho(0, W, H, 0) and viewport (0, 0, W, H) (main.cpp:497-511);
+    a point whose centre leaves the clip volume is culled; a size-S point
+    covers the fragments whose centres lie in the S x S square around its
+    window position (spec 14.4); gl_PointCoord = (0.5 + (xc - xw) / S,
+    0.5 - (yc - yw) / S) (origin UPPER_LEFT); density_frag.glsl:9-19 weights.
+    Output rows are window rows (row 0 at the bottom)."""
+    H, W = h.shape[:2]
+    out = np.zeros((H, W, 4))
+    for py in range(H):
+        for px in range(W):
+            c = h[py, px].astype(np.float64)
+            if c[3] == 0.0:
+                continue
+            r = max(rmin, min(R, int(R / c[3] ** curve)))
+            hw = 1.0 / (4.0 * r + 2.0)
+            ox, oy = px - hw, py + hw                    # object position
+            xw, yw = ox, H - oy                           # window position (ortho with y flipped)
+            if not (0.0 <= xw <= W and 0.0 <= yw <= H):
+                continue
+            S = 2 * r + 1
+            for fy in range(int(np.floor(yw - S / 2)) - 1, int(np.ceil(yw + S / 2)) + 1):
+                for fx in range(int(np.floor(xw - S / 2)) - 1, int(np.ceil(xw + S / 2)) + 1):
+                    xc, yc = fx + 0.5, fy + 0.5
+                    if not (abs(xc - xw) <= S / 2 and abs(yc - yw) <= S / 2):
+                        continue
+                    if not (0 <= fx < W and 0 <= fy < H):
+                        continue
+                    if r == 0:
+                        out[fy, fx] += c
+                        continue
+                    s, t = 0.5 + (xc - xw) / S, 0.5 - (yc - yw) / S
+                    nx, ny = 2.0 * (s + hw) - 1.0, 2.0 * (t - hw) - 1.0
+                    d = nx * nx + ny * ny
+                    if d > 1.0:
+                        continue
+                    out[fy, fx] += c * (1.0 - d) * (0.63661977236 / (r * r))
+    return out
+
+
+@pytest.mark.parametrize("R,rmin", [(0, 0), (1, 0), (3, 0), (9, 2)])
+def test_density_stamp_matches_gl_point_rasterization(orc, R, rmin):
+    """Pins the oracle's stamp geometry (sub-pixel offset, centre texel,
+    chord ends, weights, clipping at the borders and the culled column 0)
+    against the point rasterisation restated from the GL rules."""
+    rng = np.random.default_rng(40 + R)
+    W, H = 2 * R + 13, 2 * R + 11
+    h = np.zeros((H, W, 4), np.float32)
+    for py, px in [(0, 0), (0, 1), (H - 1, W - 1), (H // 2, W // 2), (1, W - 2), (H - 2, 3)]:
+        a = float(rng.choice([1.0, 3.0, 50.0]))
+        h[py, px] = [rng.uniform(0, a), rng.uniform(0, a), rng.uniform(0, a), a]
+    got = orc.density(h, R, rmin, 0.4, fp32=False)
+    ref = _gl_point_splat(h, R, rmin, 0.4)
+    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    assert (got[..., 3] > 0).sum() == (ref[..., 3] > 0).sum() > 0
+
+
 def test_density_radius_shrinks_with_density(orc):
     W = H = 41
     widths = []
