@@ -8,8 +8,9 @@ normalisation) is scored by PSNR against the same reference image.  The
 wall clock of a method at a step is warmup + all draws so far + that
 method's post-processing of the step's histogram (the DE kernel, or the
 denoiser's input build and inference); scoring is not timed, and neither is
-the flame program's compilation (a first warmup of the same flame structure
-runs before the clock and is reported).  The plain
+the flame program's compilation and the sizing of the context's buffers (a
+first warmup and draw of the same flame structure run before the clock and
+are reported).  The plain
 method is the reference renderer's (src/main.cpp:490-515 density
 estimation after draw_to_bins).
 """
@@ -77,14 +78,18 @@ def quality_curve(ctx: Context, wl, ref, window: Sequence[int], spp: Sequence[in
     ctx.density_estimate(hist, dep, de.data_ptr())
     torch.cuda.synchronize()
 
-    # A first flame of this structure compiles its flame program (hiprtc):
-    # reported as cold_warmup_s, kept out of the curve, which starts from a
-    # second fresh flame.
+    # A first flame of this structure compiles its flame program (hiprtc) and,
+    # with a two-pass draw, sizes the context's record buffers: reported as
+    # cold_warmup_s, kept out of the curve, which starts from a second fresh
+    # flame on the warm context (a renderer serving many flames pays these
+    # once).
     fh = ctx.flame(f)
     t0 = time.perf_counter()
     fh.warmup(wl.warmup_passes)
+    fh.draw_to_bins(hist, 2)
     torch.cuda.synchronize()
     cold = time.perf_counter() - t0
+    hist_t.zero_()
     compile_s = fh.program_info()[1]
     fh.close()
     fh = ctx.flame(f)

@@ -511,6 +511,16 @@ static bool sort_eligible(const rf_flame* f, uint32_t W, uint32_t H) {
   return true;
 }
 
+// Passes per sort sub-draw: a draw_to_bins call of more passes runs as
+// several chaos launches, each followed by its record passes, that share the
+// record buffers; at most 64 passes and 2^28 samples per launch, so the
+// buffers stay bounded (about 10 bytes per sample of one sub-draw) and are
+// sized once per context instead of growing with every longer draw.
+static uint32_t sort_subdraw_passes(uint64_t P) {
+  const uint64_t by_samples = P ? ((uint64_t)1 << 28) / P : 1;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, by_samples));
+}
+
 // Capacities of the next sort draw from the last one's distribution (20 %
 // headroom).  Without one, from the samples the histogram already holds (the
 // first draw in AUTO mode is an LDS draw into it), else 3x the mean.  Uploads
@@ -619,8 +629,14 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
   for (const auto& sg : segs) jobs.push_back(sg.second);
   const uint32_t n_segs = (uint32_t)segs.size();
   meta.insert(meta.end(), jobs.begin(), jobs.end());
-  if (ctx->sort_cbuf.bytes < cb * 4) HIPCHK(ctx, ctx->sort_cbuf.alloc(cb * 4));
-  if (ctx->sort_tbuf.bytes < ct * 4) HIPCHK(ctx, ctx->sort_tbuf.alloc(ct * 4));
+  // grow to at least what a full sub-draw with measured fractions needs (1.2 S
+  // plus the per-bucket and per-tile minimums), so the buffers are allocated
+  // once per context rather than at every longer draw
+  const uint64_t s_cap = ctx->sp.total_particles * (uint64_t)sort_subdraw_passes(ctx->sp.total_particles);
+  const uint64_t cb_min = std::min<uint64_t>((uint64_t)(1.25 * (double)s_cap) + 2052ull * nb, 0xfffffffcull);
+  const uint64_t ct_min = std::min<uint64_t>((uint64_t)(1.25 * (double)s_cap) + 516ull * nt_used, 0xfffffffcull);
+  if (ctx->sort_cbuf.bytes < cb * 4) HIPCHK(ctx, ctx->sort_cbuf.alloc(std::max(cb, cb_min) * 4));
+  if (ctx->sort_tbuf.bytes < ct * 4) HIPCHK(ctx, ctx->sort_tbuf.alloc(std::max(ct, ct_min) * 4));
   if (ctx->sort_bcur.bytes < (size_t)nb * 4) HIPCHK(ctx, ctx->sort_bcur.alloc((size_t)rfk::SORT_MAX_BUCKETS * 4));
   if (ctx->sort_tcur.bytes < (size_t)nt * 4)
     HIPCHK(ctx, ctx->sort_tcur.alloc((size_t)rfk::SORT_MAX_BUCKETS * rfk::SORT_TILES_PER_BUCKET * 4));
@@ -639,6 +655,8 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
 
 // After a sort draw: record where the samples went and how many overflowed.
 // bc, tc: the draw's bucket and tile cursors, already copied to the host.
+// S and drawn: the samples attempted and binned by the draw whose cursors
+// these are (the last sub-draw of a long call).
 static void sort_finish(rf_flame* f, const rfk::SortArgs& a, uint64_t S, uint64_t drawn,
                         const std::vector<uint32_t>& meta, const uint32_t* bc, const uint32_t* tc) {
   const uint32_t* bcap = meta.data() + a.n_buckets;
@@ -736,13 +754,20 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     if (hm_run == RF_HIST_LDS && !fixed_point_ok(f->desc)) hm_run = RF_HIST_QUAD;
     std::vector<uint32_t> meta;
     uint32_t max_bcap = 0;
-    const uint64_t S = P * (uint64_t)num_iter;
+    // the sort path runs long calls as sub-draws of at most `sub` passes,
+    // all planned with the capacities of a full sub-draw
+    const uint32_t sub = hm_run == RF_HIST_SORT ? std::min(num_iter, sort_subdraw_passes(P)) : num_iter;
     if (hm_run == RF_HIST_SORT && num_iter) {
-      const int rc2 = sort_prepare(ctx, f, (const float4*)h->ptr, h->w * h->h, S, a.sort, max_bcap, meta);
+      const int rc2 = sort_prepare(ctx, f, (const float4*)h->ptr, h->w * h->h, P * (uint64_t)sub, a.sort, max_bcap,
+                                   meta);
       if (rc2) return rc2;
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    if (num_iter) {
+    uint32_t last = num_iter;
+    for (uint32_t done = 0; done < num_iter; done += last) {
+      last = std::min(sub, num_iter - done);
+      if (done) a.launch = ctx->launch++;
+      a.n_iter = last;
       if (hm_run == RF_HIST_SORT) HIPCHK(ctx, rfk::launch_sort_reset(a.sort, ctx->stream));
       const int rc3 = fast_launch(ctx, f, a, P, hm_run);
       if (rc3) return rc3;
@@ -754,7 +779,7 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     sort_used = hm_run == RF_HIST_SORT && num_iter;
     sort_args = a.sort;
     sort_meta.swap(meta);
-    sort_S = S;
+    sort_S = P * (uint64_t)last;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   // counter slots, then the sort cursors, copied back with one synchronisation
@@ -775,7 +800,12 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     cnt[0] += slots[k * rfk::COUNTER_STRIDE];
     cnt[1] += slots[k * rfk::COUNTER_STRIDE + 1];
   }
-  if (sort_used) sort_finish(f, sort_args, sort_S, cnt[0], sort_meta, cur_host, cur_host + sort_args.n_buckets);
+  if (sort_used) {
+    // the cursors are the last sub-draw's: its share of the binned samples
+    const uint64_t S_all = P * (uint64_t)num_iter;
+    const uint64_t d_last = S_all ? (uint64_t)((double)cnt[0] * (double)sort_S / (double)S_all) : 0;
+    sort_finish(f, sort_args, sort_S, d_last, sort_meta, cur_host, cur_host + sort_args.n_buckets);
+  }
   if (ctx->sp.engine == RF_ENGINE_FAST && hm == RF_HIST_LDS && cnt[0]) {
     f->last_lds_hit_fraction = (double)cnt[1] / (double)cnt[0];
     if (ctx->sp.hist_mode == RF_HIST_AUTO && f->last_lds_hit_fraction < kAutoSortMaxHits) {

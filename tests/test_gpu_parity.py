@@ -128,6 +128,38 @@ def test_fast_bit_exact_vs_oracle_replica(rf, orc, gpu_available, hist_mode):
         np.testing.assert_allclose(g_hist[..., :3], bins[..., :3], rtol=1e-5, atol=1e-5)
 
 
+def test_fast_sort_long_draw_runs_as_subdraws(rf, orc, gpu_available):
+    """A sort-mode draw_to_bins call of more than 64 passes runs as chaos
+    launches of at most 64 passes (each with the next launch key) that share
+    the record buffers: bit-exact against the replica run the same way."""
+    P, T, W, H, seed = 1 << 13, 1, 192, 160, 19
+    fl = _linear_flame(rf, 29, W=W, H=H)
+    ctx = _ctx(rf, P, T, rf.RF_ENGINE_FAST, hist_mode=rf.RF_HIST_SORT, seed=seed)
+    fh = ctx.flame(fl)
+    hist = ctx.histogram(W, H)
+    fh.warmup(2)
+    drawn = fh.draw_to_bins(hist, 150)                 # sub-draws of 64, 64 and 22 passes
+    g_hist = hist.download()
+    g_pos = fh.particles()
+    perms = ctx.fast_perms(4)
+    fc = orc.flame_to_c(fl)
+    ss = orc.screen_affine(fl, W, H)
+    pos = np.zeros((P, 4), np.float32)
+    orc.fast_run(fc, P // T, P, 4, perms, seed, 0, 3, True, False, W, H, ss, pos)
+    hits = np.zeros((H, W), np.uint32)
+    o_drawn = 0
+    for launch, n in ((1, 64), (2, 64), (3, 22)):
+        o_drawn += orc.fast_run(fc, P // T, P, 4, perms, seed, launch, n, False, True, W, H, ss, pos, None, hits)
+    assert drawn == o_drawn > 0
+    assert np.array_equal(g_pos.view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(g_hist[..., 3].astype(np.uint32), hits)
+    # the next call plans from the measured distribution: exact in count, and
+    # every record fits its capacity
+    d2 = fh.draw_to_bins(hist, 70)
+    assert int(hist.download()[..., 3].sum()) == drawn + d2
+    assert fh.sort_overflow() == 0.0
+
+
 # ------------------------------------------------- non-finite particle policy
 def _nan_xform(rf, kind):
     """An xform whose output is never finite: a NaN variation weight, or an
