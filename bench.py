@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-probe", action="store_true")
     ap.add_argument("--no-psnr", action="store_true", help="skip the time-to-40-dB measurement")
+    ap.add_argument("--psnr-only", action="store_true",
+                    help="run only the time-to-40-dB measurement (cfg5) and print its JSON (for profiling)")
     ap.add_argument("--no-programs", action="store_true",
                     help="run the precompiled interpreter instead of the flame's compiled program")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -309,6 +311,9 @@ def main():
         return traffic_child(rf, args)
     if args.config == "cfg4":
         return bench_batch(rf, args)
+    if args.psnr_only:
+        print(json.dumps(time_to_40db(rf, args)), flush=True)
+        return
 
     wl = configs.WORKLOADS[args.config]()
     assert torch.cuda.is_available(), "bench.py needs the GPU"
