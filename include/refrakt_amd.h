@@ -227,7 +227,10 @@ void* rf_hist_device_ptr(rf_hist* hist);
  * seeded from the Hammersley points, then num_passes passes; nothing binned. */
 int rf_warmup(rf_ctx* ctx, rf_flame* flame, uint32_t num_passes, float tss_width);
 /* ≙ flame::draw_to_bins (src/flame.cpp:283-330): num_iter binned passes;
- * *drawn receives the number of binned samples (may be NULL). */
+ * *drawn receives the number of binned samples (may be NULL).  With the fast
+ * engine's sort histogram a call of more than 64 passes (or 2^28 samples)
+ * runs as several launches of at most that many passes, each with its own
+ * RNG launch key: the same result as drawing them in separate calls. */
 int rf_draw_to_bins(rf_ctx* ctx, rf_flame* flame, rf_hist* bins,
                     uint32_t num_iter, uint64_t* drawn);
 /* Compat-engine variants with an explicit shuffle-row schedule: rows holds
