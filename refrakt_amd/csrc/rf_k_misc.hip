@@ -175,6 +175,12 @@ __global__ __launch_bounds__(256) void k_de_radius(DEArgs g) {
 // row and, per radius, only the columns within r of its own: every source
 // it reads contributes (up to the disc test inside the square).
 constexpr int DE_TW = 32;
+// Row strides of the per-radius tables, powers of two so that indexing by
+// radius is a shift (a multiply by 2R + 1 or by the tile height is a
+// quarter-rate integer instruction in the innermost (row, radius) loop):
+// s_e rows of DE_ES >= TH + 2R entries, s_jx rows of DE_JS >= 2R + 1.
+constexpr int DE_ES_SHIFT = 6, DE_JS_SHIFT = 5;
+constexpr int DE_ES = 1 << DE_ES_SHIFT, DE_JS = 1 << DE_JS_SHIFT;
 
 template <int TH>
 __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
@@ -183,13 +189,15 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
   const int SW = DE_TW + 2 * R, SH = TH + 2 * R;
   float4* s_col = reinterpret_cast<float4*>(smem);
   unsigned long long* s_e = reinterpret_cast<unsigned long long*>(smem + sizeof(float4) * SW * SH);  // [r][row]
-  uint32_t* s_rp = reinterpret_cast<uint32_t*>(s_e + (R + 1) * SH);                                    // [row]
+  uint32_t* s_rp = reinterpret_cast<uint32_t*>(s_e + (R + 1) * DE_ES);                                 // [row]
   float* s_wt = reinterpret_cast<float*>(s_rp + SH);
-  // s_jx[r * (2R + 1) + i + R]: the columns j with a non-zero weight in row
+  // s_jx[r * DE_JS + i + R]: the columns j with a non-zero weight in row
   // offset i of radius r form the interval [lo, hi] (the disc's chord);
-  // .x = (R - hi) | (hi - lo) << 8 (mask shift relative to the texel's
-  // column and chord length - 1), .y = index of w(i, hi) in s_wt; .x = ~0u
-  // when the row has none
+  // .x = (R - hi) | (index of w(i, hi) in s_wt) << 8 (the mask shift
+  // relative to the texel's column, then the weights), .y = the chord's bit
+  // mask (2 << (hi - lo)) - 1, 0 when the row has none.  One 8-byte entry,
+  // read beside the source mask it applies to, so neither read waits on the
+  // other; 8 bytes keep three workgroups per CU at R = 9.
   const size_t jx_off = ((size_t)(reinterpret_cast<unsigned char*>(s_wt + de_wt_off(R + 1)) - smem) + 7) & ~(size_t)7;
   uint2* s_jx = reinterpret_cast<uint2*>(smem + jx_off);
   __shared__ int s_rmax;
@@ -197,22 +205,22 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
   const int tid = threadIdx.x;
   if (tid == 0) s_rmax = 0;
   for (int q = tid; q < de_wt_off(R + 1); q += 256) s_wt[q] = g.wt[q];
-  for (int q = tid; q < (R + 1) * SH; q += 256) s_e[q] = 0ull;
+  for (int q = tid; q < (R + 1) * DE_ES; q += 256) s_e[q] = 0ull;
   for (int q = tid; q < SH; q += 256) s_rp[q] = 0u;
-  for (int q = tid; q < (R + 1) * (2 * R + 1); q += 256) {
-    const int r = q / (2 * R + 1), i = q % (2 * R + 1) - R;
-    uint32_t v;
+  for (int q = tid; q < (R + 1) * DE_JS; q += 256) {
+    const int r = q >> DE_JS_SHIFT, i = (q & (DE_JS - 1)) - R;
+    uint2 v = make_uint2(0u, 0u);
     if (i >= -r && i <= r) {
       const float* w = g.wt + de_wt_off(r) + (i + r) * (2 * r + 1);
       int lo = r + 1, hi = -r - 1;
       for (int j = -r; j <= r; ++j)
         if (w[j + r] != 0.0f) { lo = min(lo, j); hi = max(hi, j); }
-      if (lo <= hi) v = (uint32_t)(R - hi) | ((uint32_t)(hi - lo) << 8);
-      else v = ~0u;
-      s_jx[q] = make_uint2(v, (uint32_t)(de_wt_off(r) + (i + r) * (2 * r + 1) + hi + r));
-    } else {
-      s_jx[q] = make_uint2(~0u, 0u);
+      // a chord is at most 2r + 1 <= 31 columns wide: its mask fits 32 bits
+      if (lo <= hi)
+        v = make_uint2((uint32_t)(R - hi) | ((uint32_t)(de_wt_off(r) + (i + r) * (2 * r + 1) + hi + r) << 8),
+                       (2u << (hi - lo)) - 1u);
     }
+    s_jx[q] = v;
   }
   __syncthreads();
   int rloc = 0;
@@ -225,7 +233,7 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
     const int r = g.rad[b];
     if (r < 0) continue;
     s_col[k] = g.hist[b];
-    atomicOr(&s_e[r * SH + ly], 1ull << lx);
+    atomicOr(&s_e[(r << DE_ES_SHIFT) + ly], 1ull << lx);
     atomicOr(&s_rp[ly], 1u << r);
     rloc = max(rloc, r);
   }
@@ -252,12 +260,11 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
       while (radii) {
         const int r = ai + __builtin_ctz(radii);
         radii &= radii - 1u;
-        const uint2 jx = s_jx[r * (2 * R + 1) + i + R];
-        if (jx.x == ~0u) continue;
+        const uint2 jx = s_jx[(r << DE_JS_SHIFT) + i + R];
+        const unsigned long long e = s_e[(r << DE_ES_SHIFT) + rowi];
         const int c0 = lx + (int)(jx.x & 0xffu);  // columns c0 .. c0 + hi - lo: j = hi .. lo
-        // a chord is at most 2r + 1 <= 29 columns wide: the mask fits 32 bits
-        uint32_t m = (uint32_t)(s_e[r * SH + rowi] >> c0) & ((2u << (jx.x >> 8)) - 1u);
-        const float* w = s_wt + jx.y;  // w[-b]: j = hi - b
+        uint32_t m = (uint32_t)(e >> c0) & jx.y;
+        const float* w = s_wt + (jx.x >> 8);  // w[-b]: j = hi - b
         const int base = rowi * SW + c0;
         while (m) {
           const int b = __builtin_ctz(m);
@@ -281,8 +288,8 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
 template <int TH>
 size_t de_tiled_lds(int R) {
   const size_t SW = DE_TW + 2 * R, SH = TH + 2 * R;
-  return SW * SH * sizeof(float4) + (size_t)(R + 1) * SH * 8 + SH * 4 + de_weight_bytes(R) + 8 +
-         (size_t)(R + 1) * (2 * R + 1) * 8;
+  return SW * SH * sizeof(float4) + (size_t)(R + 1) * DE_ES * 8 + SH * 4 + de_weight_bytes(R) + 8 +
+         (size_t)(R + 1) * DE_JS * 8;
 }
 
 // Radii beyond the LDS tile: the same gather straight from memory.
@@ -387,6 +394,7 @@ hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s) {
   // 32 + 2R <= 64 as well), the global-memory gather beyond.
   const int span = a.R > a.rmin ? a.R : a.rmin;
   // (chords of at most 31 columns keep the tap masks in 32 bits)
+  // span <= 15 keeps 2 span + 1 <= DE_JS and TH + 2 span <= DE_ES
   const bool tall = de_tiled_lds<32>(span) <= 65536 && span <= 15;
   const bool flat = de_tiled_lds<8>(span) <= 65536 && 32 + 2 * span <= 64 && span <= 15;
   if (tall || flat) {
