@@ -182,8 +182,8 @@ constexpr int DE_TW = 32;
 constexpr int DE_ES_SHIFT = 6, DE_JS_SHIFT = 5;
 constexpr int DE_ES = 1 << DE_ES_SHIFT, DE_JS = 1 << DE_JS_SHIFT;
 
-template <int TH>
-__global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
+template <int TH, int NTH>
+__global__ __launch_bounds__(NTH) void k_de_tiled(DEArgs g) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int R = max(g.R, g.rmin);  // span
   const int SW = DE_TW + 2 * R, SH = TH + 2 * R;
@@ -204,10 +204,10 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
   const int tx0 = blockIdx.x * DE_TW, ty0 = blockIdx.y * TH;
   const int tid = threadIdx.x;
   if (tid == 0) s_rmax = 0;
-  for (int q = tid; q < de_wt_off(R + 1); q += 256) s_wt[q] = g.wt[q];
-  for (int q = tid; q < (R + 1) * DE_ES; q += 256) s_e[q] = 0ull;
-  for (int q = tid; q < SH; q += 256) s_rp[q] = 0u;
-  for (int q = tid; q < (R + 1) * DE_JS; q += 256) {
+  for (int q = tid; q < de_wt_off(R + 1); q += NTH) s_wt[q] = g.wt[q];
+  for (int q = tid; q < (R + 1) * DE_ES; q += NTH) s_e[q] = 0ull;
+  for (int q = tid; q < SH; q += NTH) s_rp[q] = 0u;
+  for (int q = tid; q < (R + 1) * DE_JS; q += NTH) {
     const int r = q >> DE_JS_SHIFT, i = (q & (DE_JS - 1)) - R;
     uint2 v = make_uint2(0u, 0u);
     if (i >= -r && i <= r) {
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
   }
   __syncthreads();
   int rloc = 0;
-  for (int k = tid; k < SW * SH; k += 256) {
+  for (int k = tid; k < SW * SH; k += NTH) {
     const int ly = k / SW, lx = k % SW;
     const int cx = tx0 - R + lx, cy = ty0 - R + ly;
     const int px = cx + 1, py = (int)g.H - 1 - cy;
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void k_de_tiled(DEArgs g) {
   const int RM = min(R, s_rmax);
   const int lx = tid % DE_TW;
 #pragma unroll 1
-  for (int ly = tid / DE_TW; ly < TH; ly += 256 / DE_TW) {
+  for (int ly = tid / DE_TW; ly < TH; ly += NTH / DE_TW) {
     const int tx = tx0 + lx, ty = ty0 + ly;
     if (tx >= (int)g.W || ty >= (int)g.H) continue;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -403,10 +403,12 @@ hipError_t launch_de_tonemap(const DEArgs& a, hipStream_t s) {
   }
   if (tall) {
     dim3 grid((a.W + DE_TW - 1) / DE_TW, (a.H + 31) / 32);
-    hipLaunchKernelGGL(k_de_tiled<32>, grid, dim3(256), de_tiled_lds<32>(span), s, a);
+    // eight waves per 32 x 32 tile: the LDS allows three tiles per CU, so
+    // 24 waves per CU hide the gather's LDS latency (12 with four waves)
+    hipLaunchKernelGGL((k_de_tiled<32, 512>), grid, dim3(512), de_tiled_lds<32>(span), s, a);
   } else if (flat) {
     dim3 grid((a.W + DE_TW - 1) / DE_TW, (a.H + 7) / 8);
-    hipLaunchKernelGGL(k_de_tiled<8>, grid, dim3(256), de_tiled_lds<8>(span), s, a);
+    hipLaunchKernelGGL((k_de_tiled<8, 256>), grid, dim3(256), de_tiled_lds<8>(span), s, a);
   } else {
     dim3 grid((a.W + 15) / 16, (a.H + 15) / 16);
     hipLaunchKernelGGL(k_de_global, grid, dim3(256), 0, s, a);
