@@ -333,7 +333,7 @@ def _sparse_hist(rng, W, H, density=0.3):
     return h
 
 
-# R <= 11: 32 x 32 tiles; 12-16: 32 x 8 tiles; beyond: the global gather;
+# R <= 11: 32 x 32 tiles; 12-15: 32 x 8 tiles; beyond: the global gather;
 # rmin > R: the splat span is rmin
 @pytest.mark.parametrize("R,rmin,curve", [(9, 0, 0.4), (3, 1, 0.6), (0, 0, 0.4), (30, 0, 0.5), (11, 0, 0.3),
                                           (13, 0, 0.45), (2, 5, 0.4)])
