@@ -15,6 +15,7 @@ plain (samples + density estimation), raw samples and the denoiser scored
 against the same oracle image (refrakt_amd/convergence.py).
 """
 import argparse
+import faulthandler
 import csv
 import glob
 import json
@@ -252,10 +253,13 @@ def bench_batch(rf, args):
     from refrakt_amd import configs
     frames = configs.cfg4()
     ctx = rf.Context(1 << 12, 1, 16, seed=12345, flame_programs=not args.no_programs)
+    print("cfg4: creating the batch", file=sys.stderr, flush=True)
     b = ctx.batch([(f.flame, f.width, f.height, f.samples) for f in frames])
+    print("cfg4: autotuning", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     cfg, tune_ms = b.tune(1.0 / 16)
     tune_s = time.perf_counter() - t0
+    print(f"cfg4: tuned {cfg} in {tune_s:.1f} s; timing", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         b.run()
     torch.cuda.synchronize()
@@ -291,6 +295,8 @@ def bench_batch(rf, args):
 
 
 def main():
+    # a crash in native code prints the Python stack of every thread
+    faulthandler.enable()
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this renderer runs on one GPU; --gpus must be 1 (a job would shard by "
