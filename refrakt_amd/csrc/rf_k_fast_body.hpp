@@ -30,12 +30,16 @@ using namespace rfd;
 // many batches before it is flushed.
 // HIST_SORT staging: the samples of SI = sort_stage_iters<N>() iterations
 // wait in LDS (one fixed slot per lane and iteration, no append atomics).  A
-// flush ranks them by bucket (LDS counting sort; each lane loads its staged
-// bins first, then issues its rank atomics back to back and keeps the ranks
-// in registers), wave 0 scans the counts, reserves one run per non-empty
-// bucket with one global atomic and clears the counts, the records are placed
-// in bucket order in LDS and written out so that neighbouring lanes store
-// neighbouring addresses.  Four barriers per flush.
+// flush ranks them within their buckets (each lane loads its staged bins
+// first, then issues its rank atomics back to back and keeps the ranks in
+// registers), one lane per non-empty bucket reserves the bucket's run with
+// one global atomic and clears its count, and every lane stores its records
+// at run start + rank.  A store instruction's lanes of one bucket hold
+// consecutive ranks, so they write one contiguous piece of the run, and the
+// pieces of a run meet in L2.  Two barriers per flush.  (Placing the records
+// in bucket order in LDS first, so that each store instruction writes one
+// contiguous span, took two more barriers and five more LDS accesses per
+// record, and 5 % longer per 7680x4320 draw.)
 template <int N>
 constexpr int sort_stage_iters() { return SORT_STAGE_SLOTS / N; }
 
@@ -51,25 +55,12 @@ constexpr int sort_stage_slots() { return sort_stage_iters<N>() * N; }
 template <int N, bool ON>
 struct SortStage {
   static constexpr int S = sort_stage_slots<N>();
-  // Each lane reads back only its own staged slots, into registers, at the
-  // start of a flush; the bucket-ordered records of the flush then reuse the
-  // memory (the flush ends with a barrier before the next stage writes).
-  // rec and bkt have a spare entry S: the placement of an empty slot writes
-  // there, and the write-out reads it for positions past the last record, so
-  // neither needs a branch around its LDS accesses.
-  union {
-    uint32_t bin[S];     // staged bin index (SORT_NONE: no sample), slot = iteration * N + lane
-    uint32_t rec[S + 1]; // bucket-ordered records
-  };
+  uint32_t bin[S];       // staged bin index (SORT_NONE: no sample), slot = iteration * N + lane
   uint8_t pid[S];        // staged palette index
-  uint8_t bkt[S + 1];    // bucket of rec[j]
   uint32_t cnt[SORT_MAX_BUCKETS];    // records per bucket in this flush (zero between flushes)
-  uint32_t off[SORT_MAX_BUCKETS];    // exclusive scan of cnt
-  uint32_t base[SORT_MAX_BUCKETS];   // record j of bucket b goes to cbuf[base[b] + j] ...
-  uint32_t lim[SORT_MAX_BUCKETS];    // ... if j < lim[b], else over the bucket's capacity
+  uint2 run[SORT_MAX_BUCKETS];       // this flush's run of bucket b: first record in cbuf, records that fit
   uint32_t boff[SORT_MAX_BUCKETS];   // bucket offsets and capacities, staged once per workgroup
   uint32_t bcap[SORT_MAX_BUCKETS];
-  uint32_t total;
 };
 template <int N>
 struct SortStage<N, false> {};
@@ -78,7 +69,6 @@ template <int N>
 __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_staged, const SortArgs& a,
                                            float* __restrict__ bins, const float4* __restrict__ pal) {
   constexpr int SI = sort_stage_iters<N>();
-  static_assert(SORT_MAX_BUCKETS % WAVE == 0, "scan segments");
   const uint32_t tid = threadIdx.x;
   // 1. rank this lane's staged samples within their buckets
   uint32_t sb[SI], rank[SI];
@@ -88,94 +78,31 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_st
   for (int k = 0; k < SI; ++k)
     rank[k] = sb[k] != SORT_NONE ? atomicAdd(&st.cnt[sb[k] >> SORT_COARSE_SHIFT], 1u) : 0u;
   lds_barrier();
-  // 2. wave 0: exclusive scan of the counts (64-bucket segments, lane = bucket
-  //    within the segment), one reservation atomic per non-empty bucket,
-  //    destinations, and the counts cleared for the next flush
-  if (tid < (uint32_t)WAVE) {
-    constexpr int SEG = SORT_MAX_BUCKETS / WAVE;
-    uint32_t v[SEG], gb[SEG];
-    // all reservation atomics first (one round trip), then the scans
-#pragma unroll
-    for (int k = 0; k < SEG; ++k) {
-      const uint32_t b = k * WAVE + tid;
-      v[k] = b < a.n_buckets ? st.cnt[b] : 0u;
-      gb[k] = v[k] ? atomicAdd(a.bcur + b, v[k]) : 0u;
+  // 2. one lane per non-empty bucket: reserve the run, clear the count
+  for (uint32_t b = tid; b < a.n_buckets; b += N) {
+    const uint32_t c = st.cnt[b];
+    if (c) {
+      const uint32_t gb = atomicAdd(a.bcur + b, c);
+      const uint32_t cap = st.bcap[b];
+      st.run[b] = make_uint2(st.boff[b] + gb, cap > gb ? cap - gb : 0u);
+      st.cnt[b] = 0u;
     }
-    uint32_t carry = 0;
-#pragma unroll
-    for (int k = 0; k < SEG; ++k) {
-      if ((uint32_t)(k * WAVE) >= a.n_buckets) break;  // uniform
-      const uint32_t b = k * WAVE + tid;
-      uint32_t inc = v[k];
-#pragma unroll
-      for (int d = 1; d < WAVE; d <<= 1) {
-        const uint32_t t = (uint32_t)__shfl_up((int)inc, d);
-        if ((int)tid >= d) inc += t;
-      }
-      const uint32_t run = carry + inc - v[k];
-      if (b < a.n_buckets) {
-        const uint32_t cap = st.bcap[b];
-        st.off[b] = run;
-        st.base[b] = st.boff[b] + gb[k] - run;
-        st.lim[b] = run + (cap > gb[k] ? cap - gb[k] : 0u);
-        st.cnt[b] = 0u;
-      }
-      carry += (uint32_t)__shfl((int)inc, WAVE - 1);
-    }
-    if (tid == 0) st.total = carry;
   }
   lds_barrier();
-  // 3. place the records in bucket order (all reads first, then the writes;
-  //    an empty slot goes to the spare entry)
-  constexpr uint32_t S = (uint32_t)SortStage<N, true>::S;
-  uint32_t pb[SI], po[SI];
+  // 3. every record to its place in its bucket's run; records beyond the
+  //    bucket's capacity go straight to the histogram.  No barrier follows:
+  //    the next stage writes only this lane's own slots, and the next flush
+  //    rewrites cnt and run only after its first barrier, which every wave
+  //    reaches after these reads.
 #pragma unroll
   for (int k = 0; k < SI; ++k) {
-    pb[k] = sb[k] != SORT_NONE ? sb[k] >> SORT_COARSE_SHIFT : 0u;
-    po[k] = st.off[pb[k]];
-    rank[k] |= (uint32_t)st.pid[k * N + tid] << 24;  // ranks stay below 2^13
+    const bool v = sb[k] != SORT_NONE;
+    const uint2 r = st.run[v ? sb[k] >> SORT_COARSE_SHIFT : 0u];
+    const uint32_t p = st.pid[k * N + tid];
+    const bool ovf = v && rank[k] >= r.y;
+    if (v && !ovf) a.cbuf[r.x + rank[k]] = ((sb[k] & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | p;
+    bin_quad(bins, sb[k], ovf, p, 1.0f, pal);
   }
-#pragma unroll
-  for (int k = 0; k < SI; ++k) {
-    const uint32_t j = sb[k] != SORT_NONE ? po[k] + (rank[k] & 0xffffffu) : S;
-    st.rec[j] = ((sb[k] & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | (rank[k] >> 24);
-    st.bkt[j] = (uint8_t)pb[k];
-  }
-  lds_barrier();
-  // 4. coalesced write-out; records beyond a bucket's capacity go straight to
-  //    the histogram.  The barrier at the end keeps the next stage's writes
-  //    (to the slots rec shares) behind every wave's reads; bkt, base, lim
-  //    and total are next written after the next flush's first barrier.
-  //    Two rounds at a time, their LDS reads issued together; positions past
-  //    the last record read the spare entry and store nothing.
-  const uint32_t total = st.total;
-  const uint32_t rounds = (total + N - 1) / N;  // uniform
-  for (uint32_t r = 0; r < rounds; r += 2) {
-    uint32_t j[2], b[2], rc[2], lim[2], base[2];
-    bool in[2], ovf[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      j[h] = (r + (uint32_t)h) * N + tid;
-      in[h] = j[h] < total;
-      const uint32_t jc = in[h] ? j[h] : S;
-      b[h] = st.bkt[jc];
-      rc[h] = st.rec[jc];
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      lim[h] = st.lim[b[h]];
-      base[h] = st.base[b[h]];
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      ovf[h] = in[h] && j[h] >= lim[h];
-      if (in[h] && !ovf[h]) a.cbuf[base[h] + j[h]] = rc[h];
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-      bin_quad(bins, (b[h] << SORT_COARSE_SHIFT) | (rc[h] >> 8), ovf[h], rc[h] & 255u, 1.0f, pal);
-  }
-  lds_barrier();
 }
 
 template <int WAVES, int HM, int LOGK, class Prog>

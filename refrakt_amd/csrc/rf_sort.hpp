@@ -22,11 +22,12 @@ constexpr int SORT_STAGE_SLOTS = 4096;                                       // 
 constexpr int SORT_CHUNK = 16384;                                            // records per rebucketing workgroup
 // Tile sums add red | green << 32 and blue | count << 32 per record with
 // 64-bit LDS atomics, colours in fixed point.  A tile of at most SORT_SEG16
-// records is summed by one workgroup at 2^16 per unit (a field then holds at
-// most 65532 * 65536 < 2^32, so no carry crosses into the next); a larger
-// tile is split into segments of at most SORT_SEG15 records, each summed by
-// its own workgroup at 2^15 per unit (131068 * 32768 < 2^32).  Each sample's
-// colour is rounded by at most 2^-17 or 2^-16.
+// records is summed by one workgroup at 2^16 - 1 per unit (a field then
+// holds at most 65532 * 65535 < 2^32, so no carry crosses into the next); a
+// larger tile is split into segments of at most SORT_SEG15 records, each
+// summed by its own workgroup at 2^15 - 1 per unit (131068 * 32767 < 2^32).
+// Each sample's colour is rounded by at most 1 / (2 * 65535) or
+// 1 / (2 * 32767), about 2^-17 and 2^-16.
 constexpr int SORT_SEG16 = 65532;
 constexpr int SORT_SEG15 = 131068;
 constexpr uint32_t SORT_NONE = 0xffffffffu;

@@ -199,8 +199,8 @@ def test_cfg4_workload_against_fp64_oracle(rf, gpu_available):
 def _sort_vs_lds(rf, b, frames):
     """Replays b in LDS mode, then in sort mode (capacities from that replay):
     the trajectories are the same, so every frame's hit counts agree exactly;
-    colours within the sort path's fixed point (2^-16 per sample) plus the
-    float adds of n partial sums into a bin (n 2^-24 |sum|)."""
+    colours within the sort path's fixed point (1 / (2 * 32767) per sample)
+    plus the float adds of n partial sums into a bin (n 2^-24 |sum|)."""
     b.set_config(8, rf.RF_HIST_LDS, 64, 20)
     b.run()
     ref = [b.histogram(i) for i in range(len(frames))]
@@ -213,7 +213,7 @@ def _sort_vs_lds(rf, b, frames):
         h = b.histogram(i)
         assert np.array_equal(h[..., 3], ref[i][..., 3]), i
         n = ref[i][..., 3:4]
-        tol = 1e-6 + n * 2.0 ** -16 + n * 2.0 ** -24 * np.abs(ref[i][..., :3])
+        tol = 1e-6 + n / 65534.0 + n * 2.0 ** -24 * np.abs(ref[i][..., :3])
         assert (np.abs(h[..., :3] - ref[i][..., :3]) <= tol).all(), i
         assert int(h[..., 3].astype(np.float64).sum()) == int(d[i])
 

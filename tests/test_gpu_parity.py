@@ -119,10 +119,10 @@ def test_fast_bit_exact_vs_oracle_replica(rf, orc, gpu_available, hist_mode):
     assert np.array_equal(g_pos.view(np.uint32), pos.view(np.uint32))
     assert np.array_equal(g_hist[..., 3].astype(np.uint32), hits)
     if hist_mode == 5:
-        # the sort path sums tile colours in fixed point, 2^16 per unit (2^15
-        # for tiles of more than 65532 records): each sample's colour is
-        # rounded by 2^-16 at most
-        tol = 1e-5 + 1e-5 * np.abs(bins[..., :3]) + hits[..., None] * 2.0 ** -16
+        # the sort path sums tile colours in fixed point, 2^16 - 1 per unit
+        # (2^15 - 1 for tiles of more than 65532 records): each sample's
+        # colour is rounded by 1 / (2 * 32767) at most
+        tol = 1e-5 + 1e-5 * np.abs(bins[..., :3]) + hits[..., None] / 65534.0
         assert (np.abs(g_hist[..., :3].astype(np.float64) - bins[..., :3]) <= tol).all()
     else:
         np.testing.assert_allclose(g_hist[..., :3], bins[..., :3], rtol=1e-5, atol=1e-5)
@@ -517,10 +517,11 @@ def test_sort_path_matches_quad(rf, gpu_available, cfg, size):
     # value; the quad path adds a hot bin's samples one by one, the sort path
     # adds per-tile partial sums, so for bins hit ~10^6 times (cfg3) the two
     # legitimately differ by ~1 %.
-    # Tiles sum colours in fixed point (2^16 per unit, 2^15 for tiles of
-    # more than 65532 records): up to 2^-16 per sample of rounding.
+    # Tiles sum colours in fixed point (2^16 - 1 per unit, 2^15 - 1 for tiles
+    # of more than 65532 records): up to 1 / (2 * 32767) per sample of
+    # rounding.
     n = hq[..., 3:4].astype(np.float64)
-    tol = 1e-3 + n * 2.0 ** -16 + np.abs(hq[..., :3]) * (2e-3 + n * 2.0 ** -23)
+    tol = 1e-3 + n / 65534.0 + np.abs(hq[..., :3]) * (2e-3 + n * 2.0 ** -23)
     assert (np.abs(hs[..., :3].astype(np.float64) - hq[..., :3]) <= tol).all()
     assert 0.0 <= o2 <= max(o1, 0.0) + 1e-9
     if cfg == "cfg2":
@@ -557,7 +558,7 @@ def test_tiny_and_ragged_histograms_every_mode(rf, gpu_available, size):
     dq, hq = out[1]
     assert dq == int(hq[..., 3].astype(np.float64).sum()) > 0
     n = hq[..., 3:4].astype(np.float64)
-    tol = 1e-3 + n * 2.0 ** -16 + np.abs(hq[..., :3]) * (2e-3 + n * 2.0 ** -23)
+    tol = 1e-3 + n / 65534.0 + np.abs(hq[..., :3]) * (2e-3 + n * 2.0 ** -23)
     for mode, (d, h) in out.items():
         assert d == int(h[..., 3].astype(np.float64).sum()), mode
         if mode == 0:
@@ -601,7 +602,7 @@ def test_sort_path_at_its_size_limit(rf, gpu_available, H):
     assert torch.equal(bq[..., 3], bs[..., 3])
     assert int(bq[..., 3].double().sum()) == dq
     n = bq[..., 3:4].double()
-    tol = 1e-3 + n * 2.0 ** -16 + bq[..., :3].double().abs() * (2e-3 + n * 2.0 ** -23)
+    tol = 1e-3 + n / 65534.0 + bq[..., :3].double().abs() * (2e-3 + n * 2.0 ** -23)
     assert bool(((bs[..., :3].double() - bq[..., :3].double()).abs() <= tol).all())
 
 
