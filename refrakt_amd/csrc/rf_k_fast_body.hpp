@@ -57,7 +57,10 @@ struct SortStage {
   static constexpr int S = sort_stage_slots<N>();
   uint32_t bin[S];       // staged bin index (SORT_NONE: no sample), slot = iteration * N + lane
   uint8_t pid[S];        // staged palette index
-  uint32_t cnt[SORT_MAX_BUCKETS];    // records per bucket in this flush (zero between flushes)
+  // records per bucket in this flush (zero between flushes), then one spare
+  // counter per lane of a wave that samples without a bin count in, so that
+  // the rank atomics need no branch and those samples do not contend
+  uint32_t cnt[SORT_MAX_BUCKETS + WAVE];
   uint2 run[SORT_MAX_BUCKETS];       // this flush's run of bucket b: first record in cbuf, records that fit
   uint32_t boff[SORT_MAX_BUCKETS];   // bucket offsets and capacities, staged once per workgroup
   uint32_t bcap[SORT_MAX_BUCKETS];
@@ -70,13 +73,20 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_st
                                            float* __restrict__ bins, const float4* __restrict__ pal) {
   constexpr int SI = sort_stage_iters<N>();
   const uint32_t tid = threadIdx.x;
-  // 1. rank this lane's staged samples within their buckets
+  // 1. rank this lane's staged samples within their buckets.  Every slot is
+  //    loaded (the slots of a partial stage beyond n_staged hold older
+  //    samples and are masked after the load), then the rank atomics are
+  //    issued back to back.
   uint32_t sb[SI], rank[SI];
 #pragma unroll
-  for (int k = 0; k < SI; ++k) sb[k] = (uint32_t)k < n_staged ? st.bin[k * N + tid] : SORT_NONE;
+  for (int k = 0; k < SI; ++k) {
+    const uint32_t v = st.bin[k * N + tid];
+    sb[k] = (uint32_t)k < n_staged ? v : SORT_NONE;
+  }
 #pragma unroll
   for (int k = 0; k < SI; ++k)
-    rank[k] = sb[k] != SORT_NONE ? atomicAdd(&st.cnt[sb[k] >> SORT_COARSE_SHIFT], 1u) : 0u;
+    rank[k] = atomicAdd(&st.cnt[sb[k] != SORT_NONE ? sb[k] >> SORT_COARSE_SHIFT
+                                                  : (uint32_t)SORT_MAX_BUCKETS + (tid & (WAVE - 1))], 1u);
   lds_barrier();
   // 2. one lane per non-empty bucket: reserve the run, clear the count
   for (uint32_t b = tid; b < a.n_buckets; b += N) {
@@ -89,19 +99,40 @@ __device__ __forceinline__ void sort_flush(SortStage<N, true>& st, uint32_t n_st
     }
   }
   lds_barrier();
-  // 3. every record to its place in its bucket's run; records beyond the
-  //    bucket's capacity go straight to the histogram.  No barrier follows:
-  //    the next stage writes only this lane's own slots, and the next flush
-  //    rewrites cnt and run only after its first barrier, which every wave
-  //    reaches after these reads.
+  // 3. every record to its place in its bucket's run (the LDS reads of four
+  //    samples issued together); records beyond the bucket's capacity go
+  //    straight to the histogram.  No barrier follows: the next stage writes
+  //    only this lane's own slots, and the next flush rewrites cnt and run
+  //    only after its first barrier, which every wave reaches after these
+  //    reads.
+  constexpr int G = SI < 4 ? SI : 4;
+  static_assert(SI % G == 0, "groups of samples");
+  bool any_ovf = false;
 #pragma unroll
-  for (int k = 0; k < SI; ++k) {
-    const bool v = sb[k] != SORT_NONE;
-    const uint2 r = st.run[v ? sb[k] >> SORT_COARSE_SHIFT : 0u];
-    const uint32_t p = st.pid[k * N + tid];
-    const bool ovf = v && rank[k] >= r.y;
-    if (v && !ovf) a.cbuf[r.x + rank[k]] = ((sb[k] & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | p;
-    bin_quad(bins, sb[k], ovf, p, 1.0f, pal);
+  for (int k0 = 0; k0 < SI; k0 += G) {
+    uint2 r[G];
+    uint32_t p[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const uint32_t q = sb[k0 + k];
+      r[k] = st.run[q != SORT_NONE ? q >> SORT_COARSE_SHIFT : 0u];
+      p[k] = st.pid[(k0 + k) * N + tid];
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const uint32_t q = sb[k0 + k];
+      const bool v = q != SORT_NONE;
+      const bool fits = v && rank[k0 + k] < r[k].y;
+      if (fits) a.cbuf[r[k].x + rank[k0 + k]] = ((q & ((1u << SORT_COARSE_SHIFT) - 1u)) << 8) | p[k];
+      // a sample that does not fit keeps rank ~0 for the pass below
+      rank[k0 + k] = v && !fits ? ~0u : 0u;
+      any_ovf = any_ovf || (v && !fits);
+    }
+  }
+  if (__builtin_amdgcn_ballot_w64(any_ovf)) {  // wave-uniform and rare
+#pragma unroll
+    for (int k = 0; k < SI; ++k)
+      bin_quad(bins, sb[k], rank[k] == ~0u, st.pid[k * N + tid], 1.0f, pal);
   }
 }
 
