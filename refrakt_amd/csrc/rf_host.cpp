@@ -50,6 +50,8 @@ hipError_t HostBuf::alloc(size_t n) {
   if (n == 0) return hipSuccess;
   hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
   if (e != hipSuccess) { p = nullptr; bytes = 0; return e; }
+  e = hipHostGetDevicePointer(&dev, p, 0);
+  if (e != hipSuccess) { release(); return e; }
   bytes = n;
   return hipSuccess;
 }
@@ -57,6 +59,7 @@ hipError_t HostBuf::alloc(size_t n) {
 void HostBuf::release() {
   if (p) (void)hipHostFree(p);
   p = nullptr;
+  dev = nullptr;
   bytes = 0;
 }
 
@@ -641,7 +644,13 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
   if (ctx->sort_tcur.bytes < (size_t)nt * 4)
     HIPCHK(ctx, ctx->sort_tcur.alloc((size_t)rfk::SORT_MAX_BUCKETS * rfk::SORT_TILES_PER_BUCKET * 4));
   if (f->sort_meta_dev.bytes < meta.size() * 4) HIPCHK(ctx, f->sort_meta_dev.alloc(meta.size() * 4 * 2));
-  HIPCHK(ctx, hipMemcpyAsync(f->sort_meta_dev.p, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  // every draw ends with a stream synchronisation, so the stage is free here;
+  // sized once for any plan up to 1 MiB (pinned allocations are slow, and a
+  // sequence of growing draws would otherwise grow it step by step)
+  HIPCHK(ctx, ctx->meta_stage.alloc(std::max<size_t>(meta.size() * 4, (size_t)1 << 20)));
+  memcpy(ctx->meta_stage.p, meta.data(), meta.size() * 4);
+  HIPCHK(ctx, hipMemcpyAsync(f->sort_meta_dev.p, ctx->meta_stage.p, meta.size() * 4, hipMemcpyHostToDevice,
+                             ctx->stream));
   bind((const uint32_t*)f->sort_meta_dev.p, meta_fixed, n_chunks, n_segs);
   f->sort_meta = meta;
   f->sort_meta_S = S;
@@ -711,7 +720,9 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
   float ss[6];
   screen_affine(f->desc, h->w, h->h, ss);
   constexpr size_t counter_bytes = 8ull * rfk::COUNTER_SLOTS * rfk::COUNTER_STRIDE;
-  HIPCHK(ctx, hipMemsetAsync(ctx->counter.p, 0, counter_bytes, ctx->stream));
+  constexpr uint32_t counter_words = (uint32_t)(counter_bytes / 8);
+  // sort draws clear the counters in their first cursor reset
+  bool counter_cleared = false;
   const int hm = ctx->sp.hist_mode == RF_HIST_AUTO ? f->auto_mode : ctx->sp.hist_mode;
   bool sort_used = false;
   rfk::SortArgs sort_args{};
@@ -721,6 +732,8 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
   if (ctx->sp.engine == RF_ENGINE_COMPAT) {
     int rc = ensure_compat(ctx);
     if (rc) return rc;
+    HIPCHK(ctx, hipMemsetAsync(ctx->counter.p, 0, counter_bytes, ctx->stream));
+    counter_cleared = true;
     std::vector<uint32_t> rows;
     if (rows_in) rows.assign(rows_in, rows_in + 2 * (size_t)num_iter);
     else next_rows(ctx, num_iter, rows);
@@ -762,13 +775,21 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
                                    meta);
       if (rc2) return rc2;
     }
+    if (hm_run != RF_HIST_SORT || !num_iter) {
+      HIPCHK(ctx, hipMemsetAsync(ctx->counter.p, 0, counter_bytes, ctx->stream));
+      counter_cleared = true;
+    }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     uint32_t last = num_iter;
     for (uint32_t done = 0; done < num_iter; done += last) {
       last = std::min(sub, num_iter - done);
       if (done) a.launch = ctx->launch++;
       a.n_iter = last;
-      if (hm_run == RF_HIST_SORT) HIPCHK(ctx, rfk::launch_sort_reset(a.sort, ctx->stream));
+      if (hm_run == RF_HIST_SORT) {
+        HIPCHK(ctx, rfk::launch_sort_reset(a.sort, ctx->stream, (unsigned long long*)ctx->counter.p,
+                                           counter_cleared ? 0u : counter_words));
+        counter_cleared = true;
+      }
       const int rc3 = fast_launch(ctx, f, a, P, hm_run);
       if (rc3) return rc3;
       if (hm_run == RF_HIST_SORT) {
@@ -782,18 +803,19 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     sort_S = P * (uint64_t)last;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-  // counter slots, then the sort cursors, copied back with one synchronisation
+  // counter slots, then the sort cursors, read back by one kernel into pinned
+  // memory behind one synchronisation
   const size_t cur_words = sort_used ? (size_t)sort_args.n_buckets + sort_args.n_tiles : 0;
-  HIPCHK(ctx, ctx->readback.alloc(counter_bytes + cur_words * 4));
+  // sized once for the largest cursor arrays
+  HIPCHK(ctx, ctx->readback.alloc(std::max<size_t>(
+      counter_bytes + cur_words * 4,
+      counter_bytes + 4ull * rfk::SORT_MAX_BUCKETS * (1 + rfk::SORT_TILES_PER_BUCKET))));
   const uint64_t* slots = static_cast<const uint64_t*>(ctx->readback.p);
   uint32_t* cur_host = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->readback.p) + counter_bytes);
-  HIPCHK(ctx, hipMemcpyAsync(ctx->readback.p, ctx->counter.p, counter_bytes, hipMemcpyDeviceToHost, ctx->stream));
-  if (sort_used) {
-    HIPCHK(ctx, hipMemcpyAsync(cur_host, sort_args.bcur, (size_t)sort_args.n_buckets * 4, hipMemcpyDeviceToHost,
-                               ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(cur_host + sort_args.n_buckets, sort_args.tcur, (size_t)sort_args.n_tiles * 4,
-                               hipMemcpyDeviceToHost, ctx->stream));
-  }
+  HIPCHK(ctx, rfk::launch_draw_readback((const unsigned long long*)ctx->counter.p, counter_words,
+                                        sort_used ? sort_args.bcur : nullptr, sort_args.n_buckets,
+                                        sort_used ? sort_args.tcur : nullptr, sort_args.n_tiles,
+                                        (uint32_t*)ctx->readback.dev, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   uint64_t cnt[2] = {0, 0};
   for (int k = 0; k < rfk::COUNTER_SLOTS; ++k) {

@@ -43,6 +43,7 @@ struct DevBuf {
 // asynchronously, so a draw's readbacks share one stream synchronisation.
 struct HostBuf {
   void* p = nullptr;
+  void* dev = nullptr;   // the same memory's address for kernels
   size_t bytes = 0;
   hipError_t alloc(size_t n);
   void release();
@@ -95,8 +96,12 @@ struct rf_ctx {
   rfh::DevBuf counter;         // u64 [drawn, samples added in the LDS bin cache]
   // RF_HIST_SORT buffers: bucket and tile records, cursors, offsets, capacities
   rfh::DevBuf sort_cbuf, sort_tbuf, sort_bcur, sort_tcur, sort_mass;
-  // a draw's readbacks: counter slots, then (sort draws) bucket and tile cursors
+  // a draw's readbacks: counter slots, then (sort draws) bucket and tile
+  // cursors, written by one kernel
   rfh::HostBuf readback;
+  // staging of a sort draw's capacity plan: an upload from pinned memory
+  // does not wait for the copy the way one from pageable memory does
+  rfh::HostBuf meta_stage;
   rfh::DevBuf probe;           // atomic probe scratch
   rf_hist* render_hist = nullptr;
   rfh::DevBuf render_img;

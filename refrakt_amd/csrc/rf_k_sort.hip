@@ -9,10 +9,25 @@ namespace rfk {
 namespace {
 using namespace detail;
 
-__global__ void k_sort_reset(SortArgs a) {
+// Clears the cursors of a sort draw and, on a call's first sub-draw, the
+// draw counters (one launch instead of a cursor reset plus a memset).
+__global__ void k_sort_reset(SortArgs a, unsigned long long* __restrict__ counter, uint32_t counter_words) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < a.n_buckets) a.bcur[i] = 0u;
   if (i < a.n_tiles) a.tcur[i] = 0u;
+  if (i < counter_words) counter[i] = 0ull;
+}
+
+// A draw's readback in one launch: the counter words, then the bucket and
+// tile cursors, stored straight into pinned host memory (out is its device
+// address), instead of one copy command per array.
+__global__ void k_draw_readback(const unsigned long long* __restrict__ counter, uint32_t counter_words,
+                                const uint32_t* __restrict__ bcur, uint32_t nb, const uint32_t* __restrict__ tcur,
+                                uint32_t nt, uint32_t* __restrict__ out) {
+  const uint32_t cw = 2u * counter_words, n = cw + nb + nt;
+  const uint32_t* c32 = reinterpret_cast<const uint32_t*>(counter);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = i < cw ? c32[i] : (i < cw + nb ? bcur[i - cw] : tcur[i - cw - nb]);
 }
 
 // One workgroup per chunk of SORT_CHUNK records of a bucket (the host lists
@@ -287,9 +302,22 @@ hipError_t launch_sort_tile_mass(const float4* bins, uint32_t n_bins, uint32_t n
   return hipGetLastError();
 }
 
-hipError_t launch_sort_reset(const SortArgs& a, hipStream_t s) {
-  const uint32_t n = a.n_tiles > a.n_buckets ? a.n_tiles : a.n_buckets;
-  hipLaunchKernelGGL(k_sort_reset, dim3((n + 255) / 256), dim3(256), 0, s, a);
+hipError_t launch_sort_reset(const SortArgs& a, hipStream_t s, unsigned long long* counter, uint32_t counter_words) {
+  uint32_t n = a.n_tiles > a.n_buckets ? a.n_tiles : a.n_buckets;
+  if (!counter) counter_words = 0;
+  if (counter_words > n) n = counter_words;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_reset, dim3((n + 255) / 256), dim3(256), 0, s, a, counter, counter_words);
+  return hipGetLastError();
+}
+
+hipError_t launch_draw_readback(const unsigned long long* counter, uint32_t counter_words, const uint32_t* bcur,
+                                uint32_t nb, const uint32_t* tcur, uint32_t nt, uint32_t* out, hipStream_t s) {
+  const uint32_t n = 2u * counter_words + (bcur ? nb : 0u) + (tcur ? nt : 0u);
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = (n + 255) / 256 < 64u ? (n + 255) / 256 : 64u;
+  hipLaunchKernelGGL(k_draw_readback, dim3(blocks), dim3(256), 0, s, counter, counter_words, bcur, bcur ? nb : 0u,
+                     tcur, tcur ? nt : 0u, out);
   return hipGetLastError();
 }
 

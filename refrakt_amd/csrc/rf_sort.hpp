@@ -53,7 +53,14 @@ struct SortArgs {
 };
 
 #ifndef __HIPCC_RTC__
-hipError_t launch_sort_reset(const SortArgs& a, hipStream_t s);
+// Clears the cursors and, when counter is given, counter_words words of it.
+hipError_t launch_sort_reset(const SortArgs& a, hipStream_t s, unsigned long long* counter = nullptr,
+                             uint32_t counter_words = 0);
+// Copies counter_words 64-bit counters, then nb bucket and nt tile cursors
+// (either array may be null) into out, 32-bit words, in one launch; out may
+// be the device address of pinned host memory.
+hipError_t launch_draw_readback(const unsigned long long* counter, uint32_t counter_words, const uint32_t* bcur,
+                                uint32_t nb, const uint32_t* tcur, uint32_t nt, uint32_t* out, hipStream_t s);
 hipError_t launch_sort_rebucket(const SortArgs& a, float* bins, const float4* palette, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, float* bins, uint32_t n_bins, const float4* palette, hipStream_t s);
 // mass[t] = sum of the count channel over tile t of a histogram (t < n_tiles).
