@@ -34,9 +34,10 @@ using namespace rfd;
 // first, then issues its rank atomics back to back and keeps the ranks in
 // registers), one lane per non-empty bucket reserves the bucket's run with
 // one global atomic and clears its count, and every lane stores its records
-// at run start + rank.  A store instruction's lanes of one bucket hold
-// consecutive ranks, so they write one contiguous piece of the run, and the
-// pieces of a run meet in L2.  Two barriers per flush.  (Placing the records
+// at run start + rank.  The LDS serves one instruction's same-address
+// atomics lane after lane, so the lanes of one bucket in a store instruction
+// write one contiguous piece of the run, and the pieces of a run meet in L2
+// (correctness needs only distinct ranks).  Two barriers per flush.  (Placing the records
 // in bucket order in LDS first, so that each store instruction writes one
 // contiguous span, took two more barriers and five more LDS accesses per
 // record, and 5 % longer per 7680x4320 draw.)
