@@ -107,28 +107,45 @@ __global__ __launch_bounds__(RB_THREADS) void k_sort_rebucket(SortArgs a, float*
   for (int m = 0; m < RB_PER; ++m)
     if (rec[m] != SORT_NONE) s_out[s_off[rec[m] >> TSH] + rank[m]] = rec[m];
   __syncthreads();
+  // Write-out, two rounds at a time: both rounds' LDS reads are issued
+  // before either store, and one ballot per pair skips the overflow path.
   const uint32_t total = s_total;
-  const uint32_t rounds = (total + RB_THREADS - 1) / RB_THREADS;  // uniform
-  for (uint32_t q = 0; q < rounds; ++q) {
-    const uint32_t j = q * RB_THREADS + tid;
-    bool ovf = false;
-    uint32_t bin = 0, pidx = 0;
-    if (j < total) {
-      const uint32_t r = s_out[j];
-      const uint32_t f = r >> TSH;
-      const uint32_t local = r >> 8;
-      const uint32_t dst = s_gb[f] + (j - s_off[f]);
-      if (dst < s_tcap[f]) {
-        a.tbuf[s_toff[f] + dst] = ((local & ((1u << SORT_TILE_SHIFT) - 1u)) << 8) | (r & 255u);
-      } else {
-        ovf = true;
-        bin = (b << SORT_COARSE_SHIFT) | local;
-        pidx = r & 255u;
+  constexpr int U = 2;
+  for (uint32_t q0 = 0; q0 * RB_THREADS < total; q0 += U) {  // uniform
+    uint32_t j[U], r[U], gb[U], off[U], cap[U], toff[U];
+    bool in[U], ovf[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      j[u] = (q0 + u) * RB_THREADS + tid;
+      in[u] = j[u] < total;
+      r[u] = s_out[in[u] ? j[u] : 0u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t f = r[u] >> TSH;
+      gb[u] = s_gb[f];
+      off[u] = s_off[f];
+      cap[u] = s_tcap[f];
+      toff[u] = s_toff[f];
+    }
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t dst = gb[u] + (j[u] - off[u]);
+      ovf[u] = in[u] && dst >= cap[u];
+      if (in[u] && !ovf[u])
+        a.tbuf[toff[u] + dst] = (((r[u] >> 8) & ((1u << SORT_TILE_SHIFT) - 1u)) << 8) | (r[u] & 255u);
+      any = any || ovf[u];
+    }
+    if (__builtin_amdgcn_ballot_w64(any)) {  // wave-uniform and rare
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t bin = ovf[u] ? (b << SORT_COARSE_SHIFT) | (r[u] >> 8) : 0u;
+        // the tile's frame's palette (batched rendering), from global memory
+        const float4* pal = a.tile_pal ? palette + a.tile_pal[bin >> SORT_TILE_SHIFT] : palette;
+        bin_quad(bins, bin, ovf[u], r[u] & 255u, 1.0f, pal);
       }
     }
-    // overflow is rare: palette from global memory (the tile's frame's, batched)
-    const float4* pal = a.tile_pal ? palette + a.tile_pal[bin >> SORT_TILE_SHIFT] : palette;
-    bin_quad(bins, bin, ovf, pidx, 1.0f, pal);
   }
 }
 
