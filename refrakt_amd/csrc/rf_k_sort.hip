@@ -107,10 +107,10 @@ __global__ __launch_bounds__(RB_THREADS) void k_sort_rebucket(SortArgs a, float*
   for (int m = 0; m < RB_PER; ++m)
     if (rec[m] != SORT_NONE) s_out[s_off[rec[m] >> TSH] + rank[m]] = rec[m];
   __syncthreads();
-  // Write-out, two rounds at a time: both rounds' LDS reads are issued
-  // before either store, and one ballot per pair skips the overflow path.
+  // Write-out, four rounds at a time: their LDS reads are issued before the
+  // first store, and one ballot per four rounds skips the overflow path.
   const uint32_t total = s_total;
-  constexpr int U = 2;
+  constexpr int U = 4;
   for (uint32_t q0 = 0; q0 * RB_THREADS < total; q0 += U) {  // uniform
     uint32_t j[U], r[U], gb[U], off[U], cap[U], toff[U];
     bool in[U], ovf[U];
