@@ -273,20 +273,36 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(SortArgs a, float4*
   __syncthreads();
   const bool shared = cap > seg;  // several segments add to this tile
   const float inv = 1.0f / one;
-  for (uint32_t i = tid; i < (uint32_t)TILE_BINS; i += TILE_THREADS) {  // uniform trip count
-    const uint32_t bin = base + i;
+  constexpr int BPT = TILE_BINS / TILE_THREADS;  // bins per thread
+  static_assert(BPT * TILE_THREADS == TILE_BINS, "bins per thread");
+  float4 v[BPT];
+  bool add[BPT];
+#pragma unroll
+  for (int k = 0; k < BPT; ++k) {
+    const uint32_t i = (uint32_t)k * TILE_THREADS + tid;
     const uint64_t bc = s_t.bc[i];
     const uint64_t rg = s_t.rg[i];
     const uint32_t c = (uint32_t)(bc >> 32);
-    const float4 v = make_float4((float)(uint32_t)rg * inv, (float)(uint32_t)(rg >> 32) * inv,
-                                 (float)(uint32_t)bc * inv, (float)c);
-    const bool add = bin < n_bins && c != 0u;
-    if (shared) {
-      bin_quad_vals(reinterpret_cast<float*>(hist), bin, add, v);
-    } else if (add) {
-      float4 h = hist[bin];
-      h.x += v.x; h.y += v.y; h.z += v.z; h.w += v.w;
-      hist[bin] = h;
+    v[k] = make_float4((float)(uint32_t)rg * inv, (float)(uint32_t)(rg >> 32) * inv, (float)(uint32_t)bc * inv,
+                       (float)c);
+    add[k] = base + i < n_bins && c != 0u;
+  }
+  if (shared) {  // workgroup-uniform
+#pragma unroll
+    for (int k = 0; k < BPT; ++k)
+      bin_quad_vals(reinterpret_cast<float*>(hist), base + (uint32_t)k * TILE_THREADS + tid, add[k], v[k]);
+  } else {
+    // the touched bins' read-modify-writes, all loads in flight before the
+    // first store
+    float4 h[BPT];
+#pragma unroll
+    for (int k = 0; k < BPT; ++k)
+      h[k] = add[k] ? hist[base + (uint32_t)k * TILE_THREADS + tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+      if (!add[k]) continue;
+      h[k].x += v[k].x; h[k].y += v[k].y; h[k].z += v[k].z; h[k].w += v[k].w;
+      hist[base + (uint32_t)k * TILE_THREADS + tid] = h[k];
     }
   }
 }
