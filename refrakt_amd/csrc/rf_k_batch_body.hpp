@@ -41,11 +41,14 @@ __device__ __forceinline__ void k_batch_body(const BatchArgs& g, const Block* __
   constexpr int N = 64 * WAVES;
   __shared__ float s_x[2][N], s_y[2][N], s_c[2][N];
   __shared__ float4 s_pal[HM == 5 ? 1 : 256];
-  __shared__ uint16_t s_perm[NPERM * N];
+  __shared__ __attribute__((aligned(16))) uint16_t s_perm[NPERM * N];
   __shared__ LdsBins<LOGK> s_bins;
   __shared__ SortStage<N, HM == 5> s_sort;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < (uint32_t)(NPERM * N); i += N) s_perm[i] = g.perms[i];
+  // 16-byte copies (the table is 16-byte aligned in device memory and in LDS)
+  static_assert((NPERM * N) % 8 == 0, "whole 16-byte pieces");
+  for (uint32_t i = tid; i < (uint32_t)(NPERM * N / 8); i += N)
+    reinterpret_cast<uint4*>(s_perm)[i] = reinterpret_cast<const uint4*>(g.perms)[i];
   if constexpr (HM == 4) lds_bins_clear(s_bins);
   if constexpr (HM == 5) {
     for (uint32_t b = tid; b < g.sort.n_buckets; b += N) {

@@ -150,13 +150,16 @@ __device__ __forceinline__ void k_fast_body(const FastArgs& g, const Block* __re
   __shared__ float4 s_pal[HM == 5 ? 1 : 256];
   // The permutation rows live in LDS: a global load inside the loop would sit
   // behind the iteration's atomics in the in-order vmcnt queue and stall.
-  __shared__ uint16_t s_perm[NPERM * N];
+  __shared__ __attribute__((aligned(16))) uint16_t s_perm[NPERM * N];
   __shared__ LdsBins<LOGK> s_bins;
   __shared__ SortStage<N, HM == 5> s_sort;
   const uint32_t tid = threadIdx.x;
   if constexpr (HM != 5)
     for (uint32_t i = tid; i < 256u; i += N) s_pal[i] = g.palette[i];
-  for (uint32_t i = tid; i < (uint32_t)(NPERM * N); i += N) s_perm[i] = g.perms[i];
+  // 16-byte copies (the table is 16-byte aligned in device memory and in LDS)
+  static_assert((NPERM * N) % 8 == 0, "whole 16-byte pieces");
+  for (uint32_t i = tid; i < (uint32_t)(NPERM * N / 8); i += N)
+    reinterpret_cast<uint4*>(s_perm)[i] = reinterpret_cast<const uint4*>(g.perms)[i];
   if constexpr (HM == 4) {
     lds_bins_clear(s_bins);
     lds_bins_palette(s_bins, g.palette);
