@@ -251,6 +251,9 @@ def bench_batch(rf, args):
     first; a step is one replay (zero, chaos game, DE + tonemap of all frames)."""
     import torch
     from refrakt_amd import configs
+    # the same order as every other path and the tests: torch's GPU state
+    # first, then the library's context
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
     frames = configs.cfg4()
     ctx = rf.Context(1 << 12, 1, 16, seed=12345, flame_programs=not args.no_programs)
     print("cfg4: creating the batch", file=sys.stderr, flush=True)
