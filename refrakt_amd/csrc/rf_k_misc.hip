@@ -136,8 +136,9 @@ __device__ __forceinline__ void de_stamp(const DEArgs& g) {
 }
 
 // Splat weights of every radius 0..R with de_weight's arithmetic: entry
-// (i + r) * (2r + 1) + (j + r) of radius r starts at de_wt_off(r) = sum over
-// q < r of (2q + 1)^2.
+// (i + r) * (2r + 1) + (r - j) of radius r (each row's columns in descending
+// order, the order in which the gather walks a chord) starts at
+// de_wt_off(r) = sum over q < r of (2q + 1)^2.
 __device__ __forceinline__ int de_wt_off(int r) { return (4 * r * r * r - r) / 3; }
 
 __global__ __launch_bounds__(256) void k_de_weights(float* __restrict__ wt, int R) {
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(256) void k_de_weights(float* __restrict__ wt, int 
     m.r = (float)r;
     m.e = rf_div_rn(1.0f, (float)S);
     m.nrm = (r > 0) ? rf_div_rn(0.63661977236f, (float)(r * r)) : 1.0f;
-    for (int q = threadIdx.x; q < S * S; q += blockDim.x) wt[de_wt_off(r) + q] = de_weight(m, q / S - r, q % S - r);
+    for (int q = threadIdx.x; q < S * S; q += blockDim.x) wt[de_wt_off(r) + q] = de_weight(m, q / S - r, r - q % S);
   }
 }
 
@@ -214,10 +215,10 @@ __global__ __launch_bounds__(NTH) void k_de_tiled(DEArgs g) {
       const float* w = g.wt + de_wt_off(r) + (i + r) * (2 * r + 1);
       int lo = r + 1, hi = -r - 1;
       for (int j = -r; j <= r; ++j)
-        if (w[j + r] != 0.0f) { lo = min(lo, j); hi = max(hi, j); }
+        if (w[r - j] != 0.0f) { lo = min(lo, j); hi = max(hi, j); }
       // a chord is at most 2r + 1 <= 31 columns wide: its mask fits 32 bits
       if (lo <= hi)
-        v = make_uint2((uint32_t)(R - hi) | ((uint32_t)(de_wt_off(r) + (i + r) * (2 * r + 1) + hi + r) << 8),
+        v = make_uint2((uint32_t)(R - hi) | ((uint32_t)(de_wt_off(r) + (i + r) * (2 * r + 1) + r - hi) << 8),
                        (2u << (hi - lo)) - 1u);
     }
     s_jx[q] = v;
@@ -264,12 +265,12 @@ __global__ __launch_bounds__(NTH) void k_de_tiled(DEArgs g) {
         const unsigned long long e = s_e[(r << DE_ES_SHIFT) + rowi];
         const int c0 = lx + (int)(jx.x & 0xffu);  // columns c0 .. c0 + hi - lo: j = hi .. lo
         uint32_t m = (uint32_t)(e >> c0) & jx.y;
-        const float* w = s_wt + (jx.x >> 8);  // w[-b]: j = hi - b
+        const float* w = s_wt + (jx.x >> 8);  // w[b]: j = hi - b
         const int base = rowi * SW + c0;
         while (m) {
           const int b = __builtin_ctz(m);
           m &= m - 1u;
-          const float wgt = w[-b];
+          const float wgt = w[b];
           const float4 c = s_col[base + b];
           acc.x = fmaf(c.x, wgt, acc.x);
           acc.y = fmaf(c.y, wgt, acc.y);
