@@ -527,10 +527,10 @@ static uint32_t sort_subdraw_passes(uint64_t P) {
 // Capacities of the next sort draw from the last one's distribution (20 %
 // headroom).  Without one, from the samples the histogram already holds (the
 // first draw in AUTO mode is an LDS draw into it), else 3x the mean.  Uploads
-// offsets and capacities and grows the record buffers.  meta layout: boff,
-// bcap, toff, tcap.
+// offsets and capacities and grows the record buffers; the plan stays in
+// f->sort_meta (layout: boff, bcap, toff, tcap, then the work lists).
 static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n_bins, uint64_t S,
-                        rfk::SortArgs& a, uint32_t& max_bcap, std::vector<uint32_t>& meta) {
+                        rfk::SortArgs& a) {
   const uint32_t nb = (uint32_t)((n_bins + (1u << rfk::SORT_COARSE_SHIFT) - 1) >> rfk::SORT_COARSE_SHIFT);
   const uint32_t nt = nb * rfk::SORT_TILES_PER_BUCKET;
   const uint32_t nt_used = (uint32_t)((n_bins + (1u << rfk::SORT_TILE_SHIFT) - 1) >> rfk::SORT_TILE_SHIFT);
@@ -555,9 +555,7 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
   // and work lists on the device still fit.
   if (!f->sort_meta.empty() && f->sort_meta_S == S && f->sort_meta_bins == n_bins && f->sort_bins == n_bins &&
       f->last_sort_overflow == 0.0) {
-    meta = f->sort_meta;
     bind((const uint32_t*)f->sort_meta_dev.p, f->sort_meta_fixed, f->sort_meta_chunks, f->sort_meta_segs);
-    max_bcap = 0;
     return RF_OK;
   }
   if (f->sort_bins != n_bins || f->sort_bfrac.size() != nb) {
@@ -583,20 +581,18 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
       }
     }
   }
-  meta.assign(2 * (size_t)nb + 2 * (size_t)nt, 0u);
+  std::vector<uint32_t> meta(2 * (size_t)nb + 2 * (size_t)nt, 0u);
   uint32_t* boff = meta.data();
   uint32_t* bcap = boff + nb;
   uint32_t* toff = bcap + nb;
   uint32_t* tcap = toff + nt;
   uint64_t cb = 0, ct = 0;
-  max_bcap = 0;
   for (uint32_t b = 0; b < nb; ++b) {
     uint64_t c = (uint64_t)std::ceil((double)S * f->sort_bfrac[b] * 1.2) + 2048;
     c = (c + 3) & ~3ull;  // every bucket's records start 16-byte aligned
     boff[b] = (uint32_t)cb;
     bcap[b] = (uint32_t)std::min<uint64_t>(c, 0x7ffffffcull);
     cb += bcap[b];
-    max_bcap = std::max(max_bcap, bcap[b]);
   }
   for (uint32_t t = 0; t < nt; ++t) {
     uint64_t c = t < nt_used ? (uint64_t)std::ceil((double)S * f->sort_tfrac[t] * 1.2) + 512 : 0;
@@ -652,13 +648,12 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
   HIPCHK(ctx, hipMemcpyAsync(f->sort_meta_dev.p, ctx->meta_stage.p, meta.size() * 4, hipMemcpyHostToDevice,
                              ctx->stream));
   bind((const uint32_t*)f->sort_meta_dev.p, meta_fixed, n_chunks, n_segs);
-  f->sort_meta = meta;
+  f->sort_meta.swap(meta);
   f->sort_meta_S = S;
   f->sort_meta_bins = n_bins;
   f->sort_meta_fixed = meta_fixed;
   f->sort_meta_chunks = n_chunks;
   f->sort_meta_segs = n_segs;
-  (void)max_bcap;
   return RF_OK;
 }
 
@@ -726,7 +721,6 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
   const int hm = ctx->sp.hist_mode == RF_HIST_AUTO ? f->auto_mode : ctx->sp.hist_mode;
   bool sort_used = false;
   rfk::SortArgs sort_args{};
-  std::vector<uint32_t> sort_meta;
   uint64_t sort_S = 0;
   if (!ctx->ev[2]) { HIPCHK(ctx, hipEventCreate(&ctx->ev[2])); HIPCHK(ctx, hipEventCreate(&ctx->ev[3])); }
   if (ctx->sp.engine == RF_ENGINE_COMPAT) {
@@ -765,14 +759,11 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     int hm_run = hm;
     if (hm_run == RF_HIST_SORT && !sort_eligible(f, h->w, h->h)) hm_run = RF_HIST_QUAD;
     if (hm_run == RF_HIST_LDS && !fixed_point_ok(f->desc)) hm_run = RF_HIST_QUAD;
-    std::vector<uint32_t> meta;
-    uint32_t max_bcap = 0;
     // the sort path runs long calls as sub-draws of at most `sub` passes,
     // all planned with the capacities of a full sub-draw
     const uint32_t sub = hm_run == RF_HIST_SORT ? std::min(num_iter, sort_subdraw_passes(P)) : num_iter;
     if (hm_run == RF_HIST_SORT && num_iter) {
-      const int rc2 = sort_prepare(ctx, f, (const float4*)h->ptr, h->w * h->h, P * (uint64_t)sub, a.sort, max_bcap,
-                                   meta);
+      const int rc2 = sort_prepare(ctx, f, (const float4*)h->ptr, h->w * h->h, P * (uint64_t)sub, a.sort);
       if (rc2) return rc2;
     }
     if (hm_run != RF_HIST_SORT || !num_iter) {
@@ -799,7 +790,6 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     }
     sort_used = hm_run == RF_HIST_SORT && num_iter;
     sort_args = a.sort;
-    sort_meta.swap(meta);
     sort_S = P * (uint64_t)last;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
@@ -826,7 +816,7 @@ static int do_draw(rf_ctx* ctx, rf_flame* f, rf_hist* h, uint32_t num_iter, cons
     // the cursors are the last sub-draw's: its share of the binned samples
     const uint64_t S_all = P * (uint64_t)num_iter;
     const uint64_t d_last = S_all ? (uint64_t)((double)cnt[0] * (double)sort_S / (double)S_all) : 0;
-    sort_finish(f, sort_args, sort_S, d_last, sort_meta, cur_host, cur_host + sort_args.n_buckets);
+    sort_finish(f, sort_args, sort_S, d_last, f->sort_meta, cur_host, cur_host + sort_args.n_buckets);
   }
   if (ctx->sp.engine == RF_ENGINE_FAST && hm == RF_HIST_LDS && cnt[0]) {
     f->last_lds_hit_fraction = (double)cnt[1] / (double)cnt[0];
