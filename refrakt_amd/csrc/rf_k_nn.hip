@@ -85,7 +85,7 @@ __device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 
 // LDS-tiled implicit GEMM.  A workgroup of 4 waves covers 4 RW image rows
 // x CT_W = 64 pixels (RW rows per wave, four 16-pixel A fragments per row) and
 // 16*NT output channels.  Per KC-channel step of Cin (KC = 32 with gfx950's
-// v_mfma_f32_16x16x32_bf16 when Cin allows, else 16 with the 16x16x16 form)
+// v_mfma_f32_16x16x32_bf16 for the wide layers, else 16 with the 16x16x16 form)
 // it stages the (4 RW + 2) x (CT_W + 2) input patch (zero halo) and the
 // step's 3x3 x 16*NT x KC weights in LDS, KC + 8 channels per pixel / weight
 // row (the pad spreads 16 consecutive rows over distinct banks and keeps
@@ -272,7 +272,7 @@ __device__ __forceinline__ void conv_store_lds(const float4_t (&acc)[RW][4][NT],
 // measured slower: 16.8 ms against 14.0 ms per 7680x4320 inference, its
 // registers no longer fitting three waves per SIMD without spills.)
 template <int NT, int KC, int RW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RW == 2 ? 3 : 1))) void k_conv3x3_lds(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RW == 2 ? (KC == 16 ? 3 : 2) : 1))) void k_conv3x3_lds(
     View in, const bf16* __restrict__ w, const float* __restrict__ bias, View out, int H, int W, int Cin, int Cout,
     int relu, int vec_out, ConvFuse fz, int nbx, int nbz, int n_tiles) {
   const View& up = fz.up;
@@ -880,13 +880,17 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
       return hipErrorInvalidValue;
   }
   if (lds) {
-    // K = 32 steps pay off where each staged step feeds 64 output channels;
-    // the narrow full-resolution layers keep the smaller K = 16 tiles, whose
-    // LDS and registers fit four workgroups per CU (measured: 32-channel
-    // layers 1.15 ms with K = 16 against 1.53 ms with K = 32 at 7680x4320),
-    // and give each wave two rows (RW = 2), whose shared input rows halve the
-    // B and cut the A fragment reads per output by a third.
-    const bool k32 = Cin % 32 == 0 && nt == 4;
+    // K = 32 steps (v_mfma_f32_16x16x32_bf16: twice the products per
+    // instruction) pay off where each staged step feeds 64 output channels or
+    // the input has 64 channels or more: the 96 -> 32 decoder layer at full
+    // resolution takes 320 us at 2048^2 with them, 389 us without.  The
+    // narrow layers with 16 or 32 input channels keep the smaller K = 16
+    // tiles, whose LDS and registers fit three workgroups per CU instead of
+    // two (K = 32 there measured 203 -> 205 and 167 -> 175 us for the 32 ->
+    // 32 layers, 118 -> 144 us for the 32 -> 16 output layer).  The narrow
+    // layers give each wave two rows (RW = 2), whose shared input rows halve
+    // the B and cut the A fragment reads per output by a third.
+    const bool k32 = Cin % 32 == 0 && (nt == 4 || Cin >= 64);
     const int rw = nt <= 2 ? 2 : 1;
     const int nbx = (W + CT_W - 1) / CT_W, nbz = Cout / (16 * nt);
     const long long tiles = (long long)nbx * N * ((H + 4 * rw - 1) / (4 * rw)) * nbz;
@@ -907,7 +911,11 @@ hipError_t conv3x3(View in, const bf16* w, const float* bias, View out, int N, i
   hipLaunchKernelGGL((k_conv3x3_lds<NT_, KC_, RW_>), dim3((unsigned)n_tiles), dim3(256), 0, s, in, w, bias, out, H, \
                      W, Cin, Cout, relu, vec_out, fz, nbx, nbz, n_tiles)
     if (k32) {
-      RF_CONV_LDS(4, 32, 1);
+      switch (nt) {
+        case 4: RF_CONV_LDS(4, 32, 1); break;
+        case 2: RF_CONV_LDS(2, 32, 2); break;
+        default: RF_CONV_LDS(1, 32, 2); break;
+      }
     } else {
       switch (nt) {
         case 4: RF_CONV_LDS(4, 16, 1); break;
