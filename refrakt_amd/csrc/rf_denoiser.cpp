@@ -131,23 +131,30 @@ int forward(rf_denoiser* d, const bf16* x, int N, int H, int W, float* y_out = n
     f.pool = pool;
     return conv_f(l, in, out, h, w, f);
   };
-  // conv l over [upsample(lo) (c_lo channels), the skip channels of `cat` from c_lo on]
+  // The skip channels of a concatenation buffer `cat` (cat_cs channels, the
+  // upsampled c_lo first): inside it for training, packed at their own
+  // channel count at inference, where no upsampled half sits beside them
+  // (whole-line writes and reads instead of a third or half of each pixel).
+  auto skip = [&](DevBuf& cat, int cat_cs, int c_lo) {
+    return fused_up ? V(cat, cat_cs - c_lo, 0) : V(cat, cat_cs, c_lo);
+  };
+  // conv l over [upsample(lo) (c_lo channels), the skip channels of `cat`]
   auto up_conv = [&](int l, View lo, int c_lo, DevBuf& cat, int cat_cs, View out, int h, int w) -> hipError_t {
     if (fused_up) {
       rfn::ConvFuse f;
       f.up = lo;
       f.Cup = c_lo;
-      return conv_f(l, V(cat, cat_cs, c_lo), out, h, w, f);
+      return conv_f(l, skip(cat, cat_cs, c_lo), out, h, w, f);
     }
     const hipError_t e = rfn::upsample2(lo, V(cat, cat_cs, 0), N, h / 2, w / 2, c_lo, s);
     return e != hipSuccess ? e : conv(l, V(cat, cat_cs), out, h, w);
   };
   HIPCHK(ctx, conv(0, vx, V(d->e0a, 32), H, W));
-  HIPCHK(ctx, conv_pool(1, V(d->e0a, 32), V(d->cat0, 96, 64), V(d->p0, 32), H, W));
+  HIPCHK(ctx, conv_pool(1, V(d->e0a, 32), skip(d->cat0, 96, 64), V(d->p0, 32), H, W));
   HIPCHK(ctx, conv(2, V(d->p0, 32), V(d->e1a, 64), H1, W1));
-  HIPCHK(ctx, conv_pool(3, V(d->e1a, 64), V(d->cat1, 192, 128), V(d->p1, 64), H1, W1));
+  HIPCHK(ctx, conv_pool(3, V(d->e1a, 64), skip(d->cat1, 192, 128), V(d->p1, 64), H1, W1));
   HIPCHK(ctx, conv(4, V(d->p1, 64), V(d->e2a, 128), H2, W2));
-  HIPCHK(ctx, conv_pool(5, V(d->e2a, 128), V(d->cat2, 384, 256), V(d->p2, 128), H2, W2));
+  HIPCHK(ctx, conv_pool(5, V(d->e2a, 128), skip(d->cat2, 384, 256), V(d->p2, 128), H2, W2));
   HIPCHK(ctx, conv(6, V(d->p2, 128), V(d->b, 256), H3, W3));
   HIPCHK(ctx, up_conv(7, V(d->b, 256), 256, d->cat2, 384, V(d->d2, 128), H2, W2));
   HIPCHK(ctx, up_conv(8, V(d->d2, 128), 128, d->cat1, 192, V(d->d1, 64), H1, W1));
