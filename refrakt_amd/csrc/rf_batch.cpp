@@ -219,7 +219,7 @@ int plan_sort(rf_batch* b, const std::vector<rfk::BatchFrame>& ft, uint64_t per_
         segs.emplace_back(std::min<uint32_t>(tcap[t] - q * (uint32_t)rfk::SORT_SEG15, rfk::SORT_SEG15), (t << 16) | q);
       }
     }
-    std::stable_sort(segs.begin(), segs.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    rfh::order_segments(segs);
     sl.n_chunks = (uint32_t)chunks.size();
     sl.n_segs = (uint32_t)segs.size();
     meta.insert(meta.end(), chunks.begin(), chunks.end());

@@ -63,6 +63,20 @@ void HostBuf::release() {
   bytes = 0;
 }
 
+void order_segments(std::vector<std::pair<uint32_t, uint32_t>>& segs) {
+  constexpr int kStep = 10;
+  constexpr uint32_t kClasses = ((uint32_t)rfk::SORT_SEG15 >> kStep) + 1;
+  static_assert(rfk::SORT_SEG16 <= rfk::SORT_SEG15, "segment sizes");
+  uint32_t start[kClasses + 1] = {0};
+  // class 0 holds the largest segments
+  auto cls = [](uint32_t n) { return kClasses - 1 - std::min<uint32_t>(n >> kStep, kClasses - 1); };
+  for (const auto& sg : segs) ++start[cls(sg.first) + 1];
+  for (uint32_t k = 0; k < kClasses; ++k) start[k + 1] += start[k];
+  std::vector<std::pair<uint32_t, uint32_t>> out(segs.size());
+  for (const auto& sg : segs) out[start[cls(sg.first)]++] = sg;
+  segs.swap(out);
+}
+
 const rf_var_info* var_info(int id) {
   if (id < 0 || id >= RF_GEN_NUM_VARIATIONS) return nullptr;
   return &k_var_table[id];
@@ -370,6 +384,7 @@ static int flame_upload(rf_flame* f, float tss_width) {
   f->animated = animated;
   f->n_blocks = animated ? T : 1u;
   f->host_block.assign(1, b);
+  f->program = nullptr;  // the structure may have changed
   HIPCHK(ctx, f->blocks.alloc(sizeof(Block) * (animated ? (size_t)T + 1 : 1)));
   const size_t pristine = animated ? T : 0;
   HIPCHK(ctx, hipMemcpyAsync((Block*)f->blocks.p + pristine, &b, sizeof(Block), hipMemcpyHostToDevice, ctx->stream));
@@ -436,9 +451,17 @@ static int fast_launch(rf_ctx* ctx, rf_flame* f, const rfk::FastArgs& a, uint64_
     std::vector<std::pair<int, int>> ks{{W, hm}};
     if (ctx->sp.hist_mode == RF_HIST_AUTO && hm == RF_HIST_LDS) ks.push_back({W, RF_HIST_SORT});
     if (ctx->sp.hist_mode == RF_HIST_AUTO && hm == RF_HIST_SORT) ks.push_back({W, RF_HIST_LDS});
-    std::string err;
-    const rfj::Program* p = rfj::get_program(f->host_block[0], ks, err);
-    if (!p) return set_err(ctx, RF_ERR_HIP, err);
+    // the flame's cached program when it has the kernels (no structure key
+    // to build and look up per launch)
+    int dev = 0;
+    HIPCHK(ctx, hipGetDevice(&dev));
+    const rfj::Program* p = f->program;
+    if (!rfj::program_has(p, dev, ks)) {
+      std::string err;
+      p = rfj::get_program(f->host_block[0], ks, err);
+      if (!p) return set_err(ctx, RF_ERR_HIP, err);
+      f->program = p;
+    }
     f->program_compile_s = rfj::program_compile_seconds(p);
     f->program_used = 1;
     HIPCHK(ctx, rfj::launch_fast(p, a, P, W, hm, ctx->stream));
@@ -624,7 +647,7 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
       segs.emplace_back(std::min<uint32_t>(tcap[t] - k * (uint32_t)rfk::SORT_SEG15, rfk::SORT_SEG15), (t << 16) | k);
     }
   }
-  std::stable_sort(segs.begin(), segs.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+  order_segments(segs);
   for (const auto& sg : segs) jobs.push_back(sg.second);
   const uint32_t n_segs = (uint32_t)segs.size();
   meta.insert(meta.end(), jobs.begin(), jobs.end());

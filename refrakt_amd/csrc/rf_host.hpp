@@ -10,6 +10,9 @@
 #include "rf_device.hpp"
 
 struct rf_ctx;
+namespace rfj {
+struct Program;
+}
 
 struct rf_var_info {
   const char* name;
@@ -68,6 +71,10 @@ std::vector<uint16_t> perm_rows(uint64_t seed, int waves);
 bool fixed_point_ok(const rf_flame_desc& d);
 // Records msg as the context's last error and returns code.
 int set_err(rf_ctx* ctx, int code, const std::string& msg);
+// Orders the tile-summing segments (records a segment can hold, job) largest
+// first, in steps of 1024 records and stable within a step, so the long
+// workgroups start early: a counting pass instead of a comparison sort.
+void order_segments(std::vector<std::pair<uint32_t, uint32_t>>& segs);
 
 }  // namespace rfh
 
@@ -146,6 +153,9 @@ struct rf_flame {
   // Flame program: the host copy of the Block (its structure selects the
   // program), whether the last fast-engine launch ran one.
   std::vector<rfd::Block> host_block;
+  // the program of host_block's structure once looked up (programs live as
+  // long as the process); cleared when the flame is uploaded again
+  const rfj::Program* program = nullptr;
   int program_used = -1;
   double program_compile_s = 0.0;
 };

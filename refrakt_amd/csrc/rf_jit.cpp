@@ -250,6 +250,14 @@ const Program* get_program(const rfd::Block& b, const std::vector<std::pair<int,
   return p;
 }
 
+bool program_has(const Program* p, int device, const std::vector<std::pair<int, int>>& kernels) {
+  if (!p || p->device != device) return false;
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (const auto& k : kernels)
+    if (!p->fns.count(k)) return false;
+  return true;
+}
+
 const Program* find_program(const rfd::Block& b, std::pair<int, int> kernel) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;

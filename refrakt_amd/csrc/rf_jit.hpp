@@ -56,6 +56,10 @@ const Program* get_program(const rfd::Block& b, const std::vector<std::pair<int,
 // has the entry point `kernel`; nullptr otherwise.  Never compiles or loads.
 const Program* find_program(const rfd::Block& b, std::pair<int, int> kernel);
 
+// True when p was built for `device` and has an entry point for every
+// (waves, hist mode) in `kernels`: a cached program pointer still serves.
+bool program_has(const Program* p, int device, const std::vector<std::pair<int, int>>& kernels);
+
 // Seconds spent compiling the program (0 if it came from the cache).
 double program_compile_seconds(const Program* p);
 
