@@ -574,10 +574,15 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
     a.n_chunks = n_chunks;
     a.n_segs = n_segs;
   };
-  // Same draw size, same histogram and no overflow last time: the capacities
-  // and work lists on the device still fit.
-  if (!f->sort_meta.empty() && f->sort_meta_S == S && f->sort_meta_bins == n_bins && f->sort_bins == n_bins &&
-      f->last_sort_overflow == 0.0) {
+  // A plan covers up to twice the draw it is made for (at most a full
+  // sub-draw), so the next draw of up to that size reuses it: the same draw
+  // again, or the next of a sequence of doubling draws.  Capacities are upper
+  // bounds, a smaller draw leaves part of them empty; a plan for more than
+  // four times the draw is replaced, so the record passes do not launch
+  // mostly empty workgroups.  Same histogram and no overflow last time: the
+  // capacities and work lists on the device still fit.
+  if (!f->sort_meta.empty() && S <= f->sort_meta_S && 4 * S >= f->sort_meta_S && f->sort_meta_bins == n_bins &&
+      f->sort_bins == n_bins && f->last_sort_overflow == 0.0) {
     bind((const uint32_t*)f->sort_meta_dev.p, f->sort_meta_fixed, f->sort_meta_chunks, f->sort_meta_segs);
     return RF_OK;
   }
@@ -604,6 +609,8 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
       }
     }
   }
+  const uint64_t S_full = ctx->sp.total_particles * (uint64_t)sort_subdraw_passes(ctx->sp.total_particles);
+  const uint64_t Sp = std::max(S, std::min(2 * S, S_full));  // the samples the plan covers
   std::vector<uint32_t> meta(2 * (size_t)nb + 2 * (size_t)nt, 0u);
   uint32_t* boff = meta.data();
   uint32_t* bcap = boff + nb;
@@ -611,14 +618,14 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
   uint32_t* tcap = toff + nt;
   uint64_t cb = 0, ct = 0;
   for (uint32_t b = 0; b < nb; ++b) {
-    uint64_t c = (uint64_t)std::ceil((double)S * f->sort_bfrac[b] * 1.2) + 2048;
+    uint64_t c = (uint64_t)std::ceil((double)Sp * f->sort_bfrac[b] * 1.2) + 2048;
     c = (c + 3) & ~3ull;  // every bucket's records start 16-byte aligned
     boff[b] = (uint32_t)cb;
     bcap[b] = (uint32_t)std::min<uint64_t>(c, 0x7ffffffcull);
     cb += bcap[b];
   }
   for (uint32_t t = 0; t < nt; ++t) {
-    uint64_t c = t < nt_used ? (uint64_t)std::ceil((double)S * f->sort_tfrac[t] * 1.2) + 512 : 0;
+    uint64_t c = t < nt_used ? (uint64_t)std::ceil((double)Sp * f->sort_tfrac[t] * 1.2) + 512 : 0;
     c = (c + 3) & ~3ull;  // every tile's records start 16-byte aligned
     toff[t] = (uint32_t)ct;
     tcap[t] = (uint32_t)std::min<uint64_t>(c, 0x7ffffffcull);
@@ -654,9 +661,8 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
   // grow to at least what a full sub-draw with measured fractions needs (1.2 S
   // plus the per-bucket and per-tile minimums), so the buffers are allocated
   // once per context rather than at every longer draw
-  const uint64_t s_cap = ctx->sp.total_particles * (uint64_t)sort_subdraw_passes(ctx->sp.total_particles);
-  const uint64_t cb_min = std::min<uint64_t>((uint64_t)(1.25 * (double)s_cap) + 2052ull * nb, 0xfffffffcull);
-  const uint64_t ct_min = std::min<uint64_t>((uint64_t)(1.25 * (double)s_cap) + 516ull * nt_used, 0xfffffffcull);
+  const uint64_t cb_min = std::min<uint64_t>((uint64_t)(1.25 * (double)S_full) + 2052ull * nb, 0xfffffffcull);
+  const uint64_t ct_min = std::min<uint64_t>((uint64_t)(1.25 * (double)S_full) + 516ull * nt_used, 0xfffffffcull);
   if (ctx->sort_cbuf.bytes < cb * 4) HIPCHK(ctx, ctx->sort_cbuf.alloc(std::max(cb, cb_min) * 4));
   if (ctx->sort_tbuf.bytes < ct * 4) HIPCHK(ctx, ctx->sort_tbuf.alloc(std::max(ct, ct_min) * 4));
   if (ctx->sort_bcur.bytes < (size_t)nb * 4) HIPCHK(ctx, ctx->sort_bcur.alloc((size_t)rfk::SORT_MAX_BUCKETS * 4));
@@ -672,7 +678,7 @@ static int sort_prepare(rf_ctx* ctx, rf_flame* f, const float4* hist, uint32_t n
                              ctx->stream));
   bind((const uint32_t*)f->sort_meta_dev.p, meta_fixed, n_chunks, n_segs);
   f->sort_meta.swap(meta);
-  f->sort_meta_S = S;
+  f->sort_meta_S = Sp;
   f->sort_meta_bins = n_bins;
   f->sort_meta_fixed = meta_fixed;
   f->sort_meta_chunks = n_chunks;
