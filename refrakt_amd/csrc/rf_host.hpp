@@ -143,8 +143,10 @@ struct rf_flame {
   uint32_t sort_bins = 0;
   std::vector<double> sort_bfrac, sort_tfrac;
   double last_sort_overflow = -1.0;
-  // The last sort draw's capacities and work lists (host copy and device
-  // copy), reused while draws keep the same size and nothing overflows.
+  // The last sort plan's capacities and work lists (host copy and device
+  // copy), made for sort_meta_S samples (up to twice the draw that made it)
+  // and reused by draws of a quarter of that up to all of it while the
+  // histogram stays the same and nothing overflows.
   std::vector<uint32_t> sort_meta;
   rfh::DevBuf sort_meta_dev;
   uint64_t sort_meta_S = 0;
